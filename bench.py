@@ -1,0 +1,198 @@
+"""Benchmark of the plain m-discriminator on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+A step is one stable partition of N = 2^28 uint32 values uniform in [0,2^31)
+into m = 8 classes (class = x & 7), inputs resident in HBM, through the C ABI
+(mdiscr_partition_u32).  Prints ONE JSON line:
+  value         whole-job Gelem/s over the K timed steps
+  roofline      the partition kernel's algorithmic 8 B/elem over its average
+                launch time (HIP events on the launch stream) vs 8 TB/s
+  cpu_baseline  the reference's own SeqMdiscr (compiled from source into
+                oracle/_ref) on a bounded sample, on this host
+plus the skewed (Zipf 1.5) variant and the per-kernel breakdown.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Gelem/s & % HBM peak, 2^28 int32 m=8; batched-segment vs per-launch"
+PEAK_HBM_GBS = 8000.0
+N = 1 << 28
+M = 8
+SEED = 0x5EED0000 + 2
+BYTES_PER_ELEM = 8  # one 4-byte read + one 4-byte write per element
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def time_steps(ctx, fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def kernel_times(ctx, fn, reps, names):
+    import torch
+    ctx.timing(True)
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    ctx.timing(False)
+    out = {}
+    for nm in names:
+        ms, cnt = ctx.timing_read(nm)
+        out[nm] = (ms / cnt) if cnt else None
+    return out
+
+
+def cpu_baseline(n_sample):
+    """The reference's own SeqMdiscr<Mod8> on a bounded sample (5 runs, min and
+    max dropped, mean of the rest, as the reference's sweep does, run.py:49-55)
+    and the OpenMP restatement on all cores."""
+    import numpy as np
+    import oracle
+    from mdiscriminator_amd import workloads as W
+    x = W.uniform_u31(n_sample, SEED)
+    res = {}
+    if oracle.ref_available():
+        ts = []
+        for _ in range(5):
+            t0 = oracle.now()
+            oracle.ref_seq_mdiscr(x, 8, specialized=True)
+            ts.append(oracle.now() - t0)
+        ts = sorted(ts)[1:-1]
+        t = sum(ts) / len(ts)
+        res = {"value": round(n_sample / t / 1e9, 4), "unit": "Gelem/s", "cores": 1,
+               "kind": "reference",
+               "sample": f"SeqMdiscr<Mod8> (cuda_code/SeqMdiscr.h, compiled from source) on "
+                         f"{n_sample} uint32 of the same uniform stream; 5 runs, min/max "
+                         f"dropped, mean {t * 1e3:.1f} ms"}
+    else:
+        ts = []
+        for _ in range(5):
+            t0 = oracle.now()
+            oracle.seq_mdiscr(x, 8, oracle.MASK)
+            ts.append(oracle.now() - t0)
+        ts = sorted(ts)[1:-1]
+        t = sum(ts) / len(ts)
+        res = {"value": round(n_sample / t / 1e9, 4), "unit": "Gelem/s", "cores": 1,
+               "kind": "port", "sample": f"oracle seq_mdiscr on {n_sample} uint32"}
+    # OpenMP restatement of OpenmpMdiscr.h:7-83 (serial tuple scan inside).
+    ncores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    ncores = max(1, min(ncores, 16))
+    n_omp = min(n_sample, 1 << 24)
+    xo = x[:n_omp]
+    ts = []
+    for _ in range(3):
+        t0 = oracle.now()
+        oracle.openmp_mdiscr(xo, 8, oracle.MASK, nthreads=ncores)
+        ts.append(oracle.now() - t0)
+    t = sorted(ts)[1]
+    res["openmp"] = {"value": round(n_omp / t / 1e9, 4), "unit": "Gelem/s", "cores": ncores,
+                     "kind": "port", "sample": f"OpenmpMdiscr restatement on {n_omp} uint32, "
+                                               f"median of 3"}
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=1 << 26)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              f"(the partition shards by an all-to-all of class runs, not measured here)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import mdiscriminator_amd as md
+    from mdiscriminator_amd import workloads as W
+
+    torch.cuda.set_device(0)
+    ctx = md.Context(0)
+    stream = torch.cuda.current_stream()
+    log("generating inputs on the GPU")
+    d_in = W.uniform_u31_torch(N, SEED, "cuda:0")
+    d_zipf = W.zipf8_values_torch(N, SEED, "cuda:0")
+    d_out = torch.empty_like(d_in)
+    d_cnt = torch.empty(M, dtype=torch.int32, device="cuda:0")
+    torch.cuda.synchronize()
+
+    def step():
+        ctx.partition_u32(d_in, d_out, md.Mod8, d_counts=d_cnt, stream=stream)
+
+    def step_zipf():
+        ctx.partition_u32(d_zipf, d_out, md.Mod8, d_counts=d_cnt, stream=stream)
+
+    log(f"timing {args.steps} steps after {args.warmup} warmup")
+    dt = time_steps(ctx, step, args.steps, args.warmup)
+    value = N * args.steps / dt / 1e9
+    ms_step = dt / args.steps * 1e3
+    kernels = ["tile_hist", "tile_scan", "partition"]
+    kt = kernel_times(ctx, step, max(args.steps, 10), kernels)
+    dtz = time_steps(ctx, step_zipf, args.steps, 2)
+    ktz = kernel_times(ctx, step_zipf, max(args.steps, 10), kernels)
+    part_ms = kt["partition"]
+    achieved = N * BYTES_PER_ELEM / (part_ms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_partition_2p28_m8.json")
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get("bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        log(f"cpu baseline on a sample of {args.cpu_sample}")
+        cpu = cpu_baseline(args.cpu_sample)
+
+    whole_gbs = N * BYTES_PER_ELEM / (ms_step * 1e-3) / 1e9
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Gelem/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": "plain partition, 2^28 uint32 uniform in [0,2^31), m=8, "
+                               "class = x & 7 (Mod8)",
+                   "n": N, "m": M, "seed": SEED, "parallelism": "1 GPU"},
+        "hbm_frac_whole_path": round(whole_gbs / PEAK_HBM_GBS, 4),
+        "roofline": {"bound": "hbm", "kernel": "partition",
+                     "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes_per_launch": N * BYTES_PER_ELEM,
+                     "avg_launch_ms": round(part_ms, 5)},
+        "kernels_ms": {k: (round(v, 5) if v else v) for k, v in kt.items()},
+        "zipf_variant": {"value": round(N * args.steps / dtz / 1e9, 3), "unit": "Gelem/s",
+                         "kernels_ms": {k: (round(v, 5) if v else v) for k, v in ktz.items()}},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

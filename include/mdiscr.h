@@ -1,0 +1,161 @@
+/*
+ * mdiscr.h -- C ABI of the MI355X (gfx950) m-discriminator.
+ *
+ * An m-discriminator is a multi-way stable partition: a discriminator maps
+ * each element to a class in [0,m); the output holds every class-0 element,
+ * then every class-1 element and so on, each class in input order, and the m
+ * class counts come back beside it.
+ *
+ * The entry points replace the reference's header-only C++ templates
+ * (asbjornvg/mdiscriminator, cuda_code/):
+ *   mdiscr<DISCR>(n, num_hwd_thds, d_in, d_out) -> counts
+ *       MdiscrHost_optimized_clean.cu.h:9-15
+ *   mdiscr<DISCR>(n, num_hwd_thds, d_in1..3, d_out1..3) -> counts (SoA)
+ *       MdiscrHost_tuples.cu.h:262-272
+ *   mdiscrSegm<DISCR>(n, in, segment_sizes, out, new_sizes)
+ *       MdiscrSegmHost.cu.h:36-42
+ *   tupleToSizeArray(counts, sizes, n)
+ *       HelpersCommon.h:295-311
+ * The compile-time discriminator policy (Mod4, Mod8, Mod<N>:
+ * HelpersCommon.h:587-596, 840-849, 1200-1210) becomes the runtime
+ * mdiscr_discr value.
+ *
+ * Conventions
+ *   - Every pointer named d_* is device memory on the context's device.
+ *   - Calls are stream-ordered on `stream` (a hipStream_t; NULL = the null
+ *     stream) unless documented as synchronous; mdiscr_sync waits.
+ *   - Inputs are never written.  Outputs must not overlap inputs.
+ *   - Errors are returned as mdiscr_status, never by exiting the process.
+ *   - A context is not thread-safe: use one per host thread (the serving
+ *     loop owns its own).  The context owns all temporary device memory, so
+ *     steady-state calls do not allocate.
+ */
+#ifndef MDISCR_H
+#define MDISCR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    MDISCR_OK = 0,
+    MDISCR_EINVAL = 1,      /* bad argument (null pointer, m out of range, ...) */
+    MDISCR_ENOMEM = 2,      /* device or host allocation failed / over budget */
+    MDISCR_EDEVICE = 3,     /* a HIP runtime call failed */
+    MDISCR_EBUSY = 4,       /* serving queue full */
+    MDISCR_EUNSUPPORTED = 5 /* valid request outside what this build handles */
+} mdiscr_status;
+
+/* Discriminator kinds: the class of a 32-bit key x. */
+typedef enum {
+    MDISCR_DISCR_MOD = 0,  /* x % m           (Mod<N>, HelpersCommon.h:1207-1209) */
+    MDISCR_DISCR_MASK = 1  /* x & (m-1), m a power of two (Mod4/Mod8) */
+} mdiscr_discr_kind;
+
+typedef struct {
+    uint32_t kind; /* mdiscr_discr_kind */
+    uint32_t m;    /* number of classes, 1 <= m <= MDISCR_MAX_CLASSES */
+} mdiscr_discr;
+
+#define MDISCR_MAX_CLASSES 64u
+/* Largest n accepted: element indices and counts are 32-bit, as in the
+ * reference (unsigned int throughout), with one tile of headroom. */
+#define MDISCR_MAX_ELEMS 0xFFFF0000ull
+
+typedef struct mdiscr_ctx mdiscr_ctx;
+
+/* ---- context --------------------------------------------------------- */
+
+/* Creates a context on HIP device `device` (the current device if < 0). */
+mdiscr_status mdiscr_ctx_create(mdiscr_ctx** out, int device);
+mdiscr_status mdiscr_ctx_destroy(mdiscr_ctx* ctx);
+/* Waits for all work the context issued on `stream`. */
+mdiscr_status mdiscr_sync(mdiscr_ctx* ctx, void* stream);
+/* Human-readable text for a status code (static storage). */
+const char* mdiscr_status_str(mdiscr_status s);
+/* Text of the last failure on this context (static per context). */
+const char* mdiscr_last_error(const mdiscr_ctx* ctx);
+
+/* ---- plain partition (MdiscrHost_optimized_clean.cu.h:9-151) ---------- */
+
+/* Stable partition of d_in[0,n) into d_out[0,n).  d_counts (device, m
+ * entries, may be NULL) receives the class counts, stream-ordered. */
+mdiscr_status mdiscr_partition_u32(mdiscr_ctx* ctx, const uint32_t* d_in,
+                                   uint32_t* d_out, uint64_t n,
+                                   mdiscr_discr d, uint32_t* d_counts,
+                                   void* stream);
+
+/* Reference-shaped synchronous call: like mdiscr<DISCR>(n, num_hwd_thds,
+ * d_in, d_out), the class counts are returned in host memory h_counts (m
+ * entries) after the device work has finished.  num_hwd_thds is accepted
+ * for signature parity and ignored (the tile geometry is chosen here). */
+mdiscr_status mdiscr_u32(mdiscr_ctx* ctx, uint64_t n, uint32_t num_hwd_thds,
+                         const uint32_t* d_in, uint32_t* d_out,
+                         mdiscr_discr d, uint32_t* h_counts);
+
+/* ---- struct-of-arrays partition (MdiscrHost_tuples.cu.h:262-429) ------ */
+
+/* Partitions ncols parallel columns by the class of the 32-bit key column
+ * d_cols[key_col]; every column moves to the same index as raw bits.
+ * d_cols / d_out_cols / col_bytes are HOST arrays of ncols entries holding
+ * device pointers / element sizes in {1,2,4,8}.  ncols <= 8. */
+mdiscr_status mdiscr_partition_soa(mdiscr_ctx* ctx, uint32_t ncols,
+                                   const void* const* d_cols,
+                                   const uint32_t* col_bytes, uint32_t key_col,
+                                   void* const* d_out_cols, uint64_t n,
+                                   mdiscr_discr d, uint32_t* d_counts,
+                                   void* stream);
+
+/* ---- segmented partition (MdiscrSegmHost.cu.h:36-230) ---------------- */
+
+/* Each segment s = [d_seg_off[s], d_seg_off[s+1]) of d_in is partitioned
+ * independently into the same range of d_out.  d_seg_off is device memory
+ * with nseg+1 entries, d_seg_off[0] = 0, d_seg_off[nseg] = n, non-decreasing.
+ * h_seg_off (optional, host copy of the same offsets) lets the call plan the
+ * work without reading them back.  d_seg_counts (device, nseg*m, may be
+ * NULL) receives the per-segment class counts. */
+mdiscr_status mdiscr_partition_segmented(mdiscr_ctx* ctx, const uint32_t* d_in,
+                                         uint32_t* d_out, uint64_t n,
+                                         const uint64_t* d_seg_off,
+                                         const uint64_t* h_seg_off,
+                                         uint64_t nseg, mdiscr_discr d,
+                                         uint32_t* d_seg_counts, void* stream);
+
+/* Reference flag-array form, mdiscrSegm(n, in, segment_sizes, out,
+ * new_sizes): d_segment_sizes[i] is the segment length at each segment start
+ * and 0 elsewhere; d_new_sizes receives, per segment, the sizes array of
+ * that segment (class count at each class start, 0 elsewhere).  Synchronous
+ * on `stream`. */
+mdiscr_status mdiscr_segm_flags(mdiscr_ctx* ctx, uint64_t n,
+                                const uint32_t* d_in,
+                                const int32_t* d_segment_sizes,
+                                uint32_t* d_out, int32_t* d_new_sizes,
+                                mdiscr_discr d, void* stream);
+
+/* ---- output format (HelpersCommon.h:295-311) --------------------------- */
+
+/* d_sizes[0,n) = 0 except d_sizes[off[k]] = counts[k] for non-empty k, from
+ * device counts d_counts[0,m). */
+mdiscr_status mdiscr_sizes_from_counts(mdiscr_ctx* ctx,
+                                       const uint32_t* d_counts, uint32_t m,
+                                       uint64_t n, uint32_t* d_sizes,
+                                       void* stream);
+
+/* ---- kernel timing (measurement support) ------------------------------ */
+
+/* When enabled, every kernel the context launches is bracketed by HIP
+ * events on its stream and its duration accumulated per kernel name. */
+mdiscr_status mdiscr_timing_enable(mdiscr_ctx* ctx, int enable);
+/* Reads (synchronising the recorded events) and resets the accumulated
+ * time of kernel `name`: total milliseconds and launch count. */
+mdiscr_status mdiscr_timing_read(mdiscr_ctx* ctx, const char* name,
+                                 double* total_ms, uint64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MDISCR_H */

@@ -1,0 +1,216 @@
+"""mdiscriminator_amd -- m-way stable partition (m-discriminator) on MI355X.
+
+Python host layer over the C ABI of libmdiscr.so (include/mdiscr.h).  It
+mirrors the reference's host interface (asbjornvg/mdiscriminator, cuda_code/):
+
+    reference                                           here
+    ------------------------------------------------    ---------------------------
+    Mod4 / Mod8 / Mod<N>  (HelpersCommon.h:587-596,     Mod4, Mod8, Mod(N), Mask(N)
+        840-849, 1200-1210)
+    mdiscr<DISCR>(n, num_hwd_thds, d_in, d_out)         mdiscr(DISCR, n, num_hwd_thds,
+        -> counts (MdiscrHost_optimized_clean.cu.h:9)       d_in, d_out) -> counts
+    mdiscr<DISCR>(n, thds, in1..3, out1..3) -> counts   mdiscr_tuples(...)
+        (MdiscrHost_tuples.cu.h:262-272)
+    mdiscrSegm<DISCR>(n, in, segment_sizes, out,        mdiscr_segm(DISCR, n, in,
+        new_sizes) (MdiscrSegmHost.cu.h:36-42)              segment_sizes, out, new_sizes)
+    tupleToSizeArray(counts, sizes, n)                  sizes_from_counts(...)
+        (HelpersCommon.h:295-311)
+
+Device buffers are torch tensors on the GPU (plumbing only: all compute runs in
+the HIP kernels of libmdiscr.so).  Errors raise MdiscrError with the library's
+status code; nothing falls back to the CPU.
+"""
+import ctypes
+from dataclasses import dataclass
+
+from . import _lib
+from ._lib import MdiscrError, DISCR_MOD, DISCR_MASK, MAX_CLASSES, MAX_ELEMS
+
+__all__ = [
+    "Discr", "Mod", "Mask", "Mod4", "Mod8", "Context", "MdiscrError", "default_context",
+    "mdiscr", "mdiscr_tuples", "mdiscr_segm", "sizes_from_counts", "MAX_CLASSES",
+]
+
+
+@dataclass(frozen=True)
+class Discr:
+    """A discriminator: class = x % m (MOD) or x & (m-1) (MASK)."""
+    kind: int
+    m: int
+
+    def c(self):
+        return _lib.MdiscrDiscr(self.kind, self.m)
+
+    def apply(self, x):
+        """Host-side class of x (numpy arrays or ints)."""
+        return (x & (self.m - 1)) if self.kind == DISCR_MASK else (x % self.m)
+
+
+def Mod(n):
+    """Mod<N>: class = x % N (HelpersCommon.h:1200-1210)."""
+    return Discr(DISCR_MOD, int(n))
+
+
+def Mask(n):
+    """class = x & (N-1), N a power of two."""
+    return Discr(DISCR_MASK, int(n))
+
+
+Mod4 = Mask(4)  # HelpersCommon.h:587-596
+Mod8 = Mask(8)  # HelpersCommon.h:840-849
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    if isinstance(t, int):
+        return t
+    return t.data_ptr()
+
+
+def _stream(stream):
+    if stream is None:
+        import torch
+        return torch.cuda.current_stream().cuda_stream
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream
+
+
+class Context:
+    """Owns an mdiscr_ctx (device workspace, timing records)."""
+
+    def __init__(self, device=-1):
+        self.lib = _lib.load()
+        h = ctypes.c_void_p()
+        st = self.lib.mdiscr_ctx_create(ctypes.byref(h), int(device))
+        if st != _lib.OK:
+            raise MdiscrError(st, "mdiscr_ctx_create: " +
+                              self.lib.mdiscr_status_str(st).decode())
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.mdiscr_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st, what):
+        if st != _lib.OK:
+            msg = self.lib.mdiscr_last_error(self.h).decode(errors="replace")
+            raise MdiscrError(st, f"{what}: {msg}")
+
+    # -- plain ---------------------------------------------------------------
+    def partition_u32(self, d_in, d_out, discr, d_counts=None, n=None, stream=None):
+        """Stream-ordered stable partition of d_in into d_out (uint32/int32 tensors)."""
+        n = d_in.numel() if n is None else n
+        st = self.lib.mdiscr_partition_u32(self.h, _ptr(d_in), _ptr(d_out), n, discr.c(),
+                                           _ptr(d_counts), _stream(stream))
+        self._check(st, "mdiscr_partition_u32")
+
+    def mdiscr(self, discr, num_elems, num_hwd_thds, d_in, d_out):
+        """Reference-shaped synchronous call; returns the m class counts."""
+        counts = (ctypes.c_uint32 * discr.m)()
+        st = self.lib.mdiscr_u32(self.h, num_elems, num_hwd_thds, _ptr(d_in), _ptr(d_out),
+                                 discr.c(), counts)
+        self._check(st, "mdiscr_u32")
+        return list(counts)
+
+    # -- struct of arrays ------------------------------------------------------
+    def partition_soa(self, cols, out_cols, discr, key_col=0, d_counts=None, n=None,
+                      stream=None):
+        """Partitions parallel columns (tensors of 1/2/4/8-byte elements) by the
+        class of cols[key_col] (a 32-bit column); payloads move as raw bits."""
+        k = len(cols)
+        n = cols[key_col].numel() if n is None else n
+        pin = (ctypes.c_void_p * k)(*[_ptr(c) for c in cols])
+        pout = (ctypes.c_void_p * k)(*[_ptr(c) for c in out_cols])
+        nb = (ctypes.c_uint32 * k)(*[c.element_size() for c in cols])
+        st = self.lib.mdiscr_partition_soa(self.h, k, pin, nb, key_col, pout, n, discr.c(),
+                                           _ptr(d_counts), _stream(stream))
+        self._check(st, "mdiscr_partition_soa")
+
+    # -- segmented -------------------------------------------------------------
+    def partition_segmented(self, d_in, d_out, d_seg_off, discr, h_seg_off=None,
+                            d_seg_counts=None, stream=None):
+        """Each segment [off[s], off[s+1]) partitioned independently."""
+        nseg = d_seg_off.numel() - 1
+        hp = None
+        if h_seg_off is not None:
+            hp = h_seg_off.ctypes.data_as(ctypes.c_void_p)
+        st = self.lib.mdiscr_partition_segmented(self.h, _ptr(d_in), _ptr(d_out), d_in.numel(),
+                                                 _ptr(d_seg_off), hp, nseg, discr.c(),
+                                                 _ptr(d_seg_counts), _stream(stream))
+        self._check(st, "mdiscr_partition_segmented")
+
+    def segm_flags(self, discr, num_elems, d_in, d_segment_sizes, d_out, d_new_sizes,
+                   stream=None):
+        st = self.lib.mdiscr_segm_flags(self.h, num_elems, _ptr(d_in), _ptr(d_segment_sizes),
+                                        _ptr(d_out), _ptr(d_new_sizes), discr.c(),
+                                        _stream(stream))
+        self._check(st, "mdiscr_segm_flags")
+
+    # -- output format ---------------------------------------------------------
+    def sizes_from_counts(self, d_counts, m, n, d_sizes, stream=None):
+        st = self.lib.mdiscr_sizes_from_counts(self.h, _ptr(d_counts), m, n, _ptr(d_sizes),
+                                               _stream(stream))
+        self._check(st, "mdiscr_sizes_from_counts")
+
+    # -- timing ----------------------------------------------------------------
+    def timing(self, enable):
+        self._check(self.lib.mdiscr_timing_enable(self.h, 1 if enable else 0), "timing_enable")
+
+    def timing_read(self, name):
+        ms = ctypes.c_double()
+        cnt = ctypes.c_uint64()
+        self._check(self.lib.mdiscr_timing_read(self.h, name.encode(), ctypes.byref(ms),
+                                                ctypes.byref(cnt)), "timing_read")
+        return ms.value, cnt.value
+
+    def sync(self, stream=None):
+        self._check(self.lib.mdiscr_sync(self.h, _stream(stream)), "mdiscr_sync")
+
+
+_default = {}
+
+
+def default_context(device=None):
+    import torch
+    dev = torch.cuda.current_device() if device is None else int(device)
+    if dev not in _default:
+        _default[dev] = Context(dev)
+    return _default[dev]
+
+
+def mdiscr(discr, num_elems, num_hwd_thds, d_in, d_out):
+    """mdiscr<DISCR>(n, num_hwd_thds, d_in, d_out) -> class counts
+    (MdiscrHost_optimized_clean.cu.h:9-151)."""
+    return default_context(d_in.device.index).mdiscr(discr, num_elems, num_hwd_thds, d_in,
+                                                     d_out)
+
+
+def mdiscr_tuples(discr, num_elems, num_hwd_thds, d_in1, d_in2, d_in3, d_out1, d_out2, d_out3):
+    """Three-column form (MdiscrHost_tuples.cu.h:262-429): class from d_in1."""
+    import torch
+    ctx = default_context(d_in1.device.index)
+    counts = torch.empty(discr.m, dtype=torch.int32, device=d_in1.device)
+    ctx.partition_soa([d_in1, d_in2, d_in3], [d_out1, d_out2, d_out3], discr, key_col=0,
+                      d_counts=counts, n=num_elems)
+    return [int(x) for x in counts.cpu().tolist()]
+
+
+def mdiscr_segm(discr, num_elems, in_array, segment_sizes, out_array, new_sizes):
+    """mdiscrSegm<DISCR>(n, in, segment_sizes, out, new_sizes)
+    (MdiscrSegmHost.cu.h:36-42)."""
+    default_context(in_array.device.index).segm_flags(discr, num_elems, in_array, segment_sizes,
+                                                      out_array, new_sizes)
+
+
+def sizes_from_counts(d_counts, m, n, d_sizes):
+    """tupleToSizeArray (HelpersCommon.h:295-311) on device counts."""
+    default_context(d_counts.device.index).sizes_from_counts(d_counts, m, n, d_sizes)

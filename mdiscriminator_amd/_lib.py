@@ -1,0 +1,78 @@
+"""ctypes binding of libmdiscr.so (the C ABI declared in include/mdiscr.h).
+
+Loading fails loudly when the library has not been built: there is no CPU
+fallback on the product path.
+"""
+import ctypes
+import os
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libmdiscr.so")
+
+OK, EINVAL, ENOMEM, EDEVICE, EBUSY, EUNSUPPORTED = range(6)
+DISCR_MOD, DISCR_MASK = 0, 1
+MAX_CLASSES = 64
+MAX_ELEMS = 0xFFFF0000
+
+
+class MdiscrDiscr(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_uint32), ("m", ctypes.c_uint32)]
+
+
+class MdiscrError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"mdiscr status {status}: {msg}")
+        self.status = status
+
+
+_vp = ctypes.c_void_p
+_u32 = ctypes.c_uint32
+_u64 = ctypes.c_uint64
+_int = ctypes.c_int
+
+# name -> (restype, argtypes); every function declared in include/mdiscr.h.
+SIGNATURES = {
+    "mdiscr_ctx_create": (_int, [ctypes.POINTER(_vp), _int]),
+    "mdiscr_ctx_destroy": (_int, [_vp]),
+    "mdiscr_sync": (_int, [_vp, _vp]),
+    "mdiscr_status_str": (ctypes.c_char_p, [_int]),
+    "mdiscr_last_error": (ctypes.c_char_p, [_vp]),
+    "mdiscr_partition_u32": (_int, [_vp, _vp, _vp, _u64, MdiscrDiscr, _vp, _vp]),
+    "mdiscr_u32": (_int, [_vp, _u64, _u32, _vp, _vp, MdiscrDiscr, _vp]),
+    "mdiscr_partition_soa": (_int, [_vp, _u32, _vp, _vp, _u32, _vp, _u64, MdiscrDiscr, _vp, _vp]),
+    "mdiscr_partition_segmented": (_int, [_vp, _vp, _vp, _u64, _vp, _vp, _u64, MdiscrDiscr, _vp,
+                                          _vp]),
+    "mdiscr_segm_flags": (_int, [_vp, _u64, _vp, _vp, _vp, _vp, MdiscrDiscr, _vp]),
+    "mdiscr_sizes_from_counts": (_int, [_vp, _vp, _u32, _u64, _vp, _vp]),
+    "mdiscr_timing_enable": (_int, [_vp, _int]),
+    "mdiscr_timing_read": (_int, [_vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
+                                  ctypes.POINTER(_u64)]),
+}
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Loads libmdiscr.so and declares its signatures (once)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(
+            f"{path} is not built; run __graft_entry__.build() or "
+            "python -m mdiscriminator_amd.build")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        if not hasattr(lib, name):
+            continue
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols(path=LIB_PATH):
+    """Names from SIGNATURES that the built library exports."""
+    lib = ctypes.CDLL(path)
+    return [n for n in SIGNATURES if hasattr(lib, n)]

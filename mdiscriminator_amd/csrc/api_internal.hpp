@@ -1,0 +1,46 @@
+// api_internal.hpp -- context state shared by the C-ABI translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/mdiscr.h"
+
+struct TimedLaunch {
+    bool on = false;
+    hipEvent_t a = nullptr, b = nullptr;
+    const char* name = nullptr;
+};
+
+struct mdiscr_ctx {
+    int device = 0;
+    int num_cu = 256;
+    void* ws = nullptr;        // grow-only workspace
+    size_t ws_bytes = 0;
+    void* small = nullptr;     // 64 KiB of device scratch for count readbacks
+    bool timing = false;
+    std::vector<TimedLaunch> pending;
+    std::map<std::string, std::pair<double, uint64_t>> timing_acc;
+    void* extra = nullptr;     // segmented-path planning buffers
+    char err[256] = {0};
+};
+
+mdiscr_status ctx_fail(mdiscr_ctx* ctx, mdiscr_status s, const char* fmt, ...);
+mdiscr_status ctx_hip(mdiscr_ctx* ctx, hipError_t e, const char* what);
+mdiscr_status ctx_workspace(mdiscr_ctx* ctx, size_t bytes, void** out);
+mdiscr_status ctx_small_buffer(mdiscr_ctx* ctx, size_t bytes, void** out);
+void ctx_release_extra(mdiscr_ctx* ctx);
+void timing_begin(mdiscr_ctx* ctx, hipStream_t s, TimedLaunch& t);
+void timing_end(mdiscr_ctx* ctx, hipStream_t s, TimedLaunch& t, const char* name);
+mdiscr_status check_discr(mdiscr_ctx* ctx, mdiscr_discr d);
+uint32_t class_bits(uint32_t m);
+
+inline bool ranges_overlap(const void* a, const void* b, uint64_t bytes) {
+    const uintptr_t x = (uintptr_t)a, y = (uintptr_t)b;
+    return x < y + bytes && y < x + bytes;
+}

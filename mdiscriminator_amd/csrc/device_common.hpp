@@ -1,0 +1,95 @@
+// device_common.hpp -- gfx950 device helpers shared by the partition kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define MDISCR_HD __host__ __device__
+#include "divmagic.h"
+
+namespace mdiscr {
+
+constexpr int kWave = 64;      // CDNA wavefront width
+constexpr int kNumXcd = 8;     // MI355X: 8 XCDs, workgroups dealt round-robin
+constexpr int kMaxClasses = 64;
+
+// ---- discriminators (the class of a 32-bit key) ---------------------------
+
+// Mod4 / Mod8 / Mod<2^k> (HelpersCommon.h:593-595, 846-848): x & (m-1).
+struct DiscrMask {
+    uint32_t mask;
+    __device__ __forceinline__ uint32_t operator()(uint32_t x) const {
+        return x & mask;
+    }
+};
+
+// Mod<N> (HelpersCommon.h:1207-1209): x % N by the invariant-divisor magic.
+struct DiscrMod {
+    mdiscr_divmagic g;
+    __device__ __forceinline__ uint32_t operator()(uint32_t x) const {
+        return mdiscr_divmagic_mod(g, x);
+    }
+};
+
+// ---- wave64 primitives ------------------------------------------------------
+
+__device__ __forceinline__ uint32_t lane_id() {
+    return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+
+// Number of set bits of `mask` in lanes strictly below this lane.
+__device__ __forceinline__ uint32_t mbcnt64(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ __forceinline__ uint64_t ballot64(bool p) {
+    return __builtin_amdgcn_ballot_w64(p);
+}
+
+// Exclusive prefix sum across the 64 lanes of a wave.
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t lane) {
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        uint32_t y = __shfl_up(x, d, kWave);
+        if (lane >= (uint32_t)d) x += y;
+    }
+    return x - v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int d = kWave / 2; d >= 1; d >>= 1) v += __shfl_xor(v, d, kWave);
+    return v;
+}
+
+// Rank of this lane's class among the lower lanes holding the same class,
+// and the number of lanes holding it, from NBITS ballots of the class bits.
+// A lane differs from another iff some class bit differs, so the mismatch
+// mask is the OR over bits of (ballot of bit b) xor (own bit b splat).
+template <int NBITS>
+__device__ __forceinline__ void wave_match_rank(uint32_t c, uint32_t lane,
+                                                uint32_t& rank, uint32_t& cnt) {
+    uint64_t mism = 0;
+#pragma unroll
+    for (int b = 0; b < NBITS; ++b) {
+        const bool bit = (c >> b) & 1u;
+        const uint64_t B = ballot64(bit);
+        mism |= bit ? ~B : B;
+    }
+    rank = lane - mbcnt64(mism);
+    cnt = kWave - (uint32_t)__popcll(mism);
+}
+
+// ---- XCD-aware tile order -------------------------------------------------------
+// Workgroup b runs on XCD b % 8.  Give each XCD one contiguous run of tiles so
+// that neighbouring tiles (which share partially written output lines at class
+// run boundaries) meet in the same L2.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nblocks) {
+    const uint32_t q = nblocks / kNumXcd, r = nblocks % kNumXcd;
+    const uint32_t x = b % kNumXcd, i = b / kNumXcd;
+    return x * q + (x < r ? x : r) + i;
+}
+
+}  // namespace mdiscr

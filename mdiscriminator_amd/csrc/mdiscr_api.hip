@@ -1,0 +1,326 @@
+// mdiscr_api.hip -- C ABI of the gfx950 m-discriminator (include/mdiscr.h).
+//
+// The context owns one device workspace that grows to the largest request
+// and is reused, so steady-state calls make no device allocation (the
+// reference cudaMallocs and frees its temporaries on every call,
+// MdiscrHost_optimized_clean.cu.h:43-45,145-147).
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mdiscr.h"
+#include "api_internal.hpp"
+#include "plain_kernels.hpp"
+
+using namespace mdiscr;
+
+// ---------------------------------------------------------------------------
+// context
+
+mdiscr_status ctx_fail(mdiscr_ctx* ctx, mdiscr_status s, const char* fmt, ...) {
+    if (ctx) {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(ctx->err, sizeof(ctx->err), fmt, ap);
+        va_end(ap);
+    }
+    return s;
+}
+
+mdiscr_status ctx_hip(mdiscr_ctx* ctx, hipError_t e, const char* what) {
+    if (e == hipSuccess) return MDISCR_OK;
+    return ctx_fail(ctx, MDISCR_EDEVICE, "%s: %s", what, hipGetErrorString(e));
+}
+
+mdiscr_status ctx_workspace(mdiscr_ctx* ctx, size_t bytes, void** out) {
+    bytes = (bytes + 255) & ~(size_t)255;
+    if (bytes > ctx->ws_bytes) {
+        if (ctx->ws) {
+            // The old workspace may still be in use by queued work.
+            hipError_t e = hipDeviceSynchronize();
+            if (e != hipSuccess) return ctx_hip(ctx, e, "hipDeviceSynchronize");
+            hipFree(ctx->ws);
+            ctx->ws = nullptr;
+            ctx->ws_bytes = 0;
+        }
+        hipError_t e = hipMalloc(&ctx->ws, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return ctx_fail(ctx, MDISCR_ENOMEM, "workspace of %zu bytes: %s", bytes,
+                            hipGetErrorString(e));
+        }
+        ctx->ws_bytes = bytes;
+    }
+    *out = ctx->ws;
+    return MDISCR_OK;
+}
+
+void timing_begin(mdiscr_ctx* ctx, hipStream_t s, TimedLaunch& t) {
+    t.on = ctx->timing;
+    if (!t.on) return;
+    hipEventCreate(&t.a);
+    hipEventCreate(&t.b);
+    hipEventRecord(t.a, s);
+}
+
+void timing_end(mdiscr_ctx* ctx, hipStream_t s, TimedLaunch& t, const char* name) {
+    if (!t.on) return;
+    hipEventRecord(t.b, s);
+    t.name = name;
+    ctx->pending.push_back(t);
+}
+
+extern "C" {
+
+mdiscr_status mdiscr_ctx_create(mdiscr_ctx** out, int device) {
+    if (!out) return MDISCR_EINVAL;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        (void)hipGetLastError();
+        return MDISCR_EDEVICE;
+    }
+    if (device < 0) {
+        if (hipGetDevice(&device) != hipSuccess) return MDISCR_EDEVICE;
+    }
+    if (device >= ndev) return MDISCR_EINVAL;
+    if (hipSetDevice(device) != hipSuccess) return MDISCR_EDEVICE;
+    mdiscr_ctx* ctx = new (std::nothrow) mdiscr_ctx();
+    if (!ctx) return MDISCR_ENOMEM;
+    ctx->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cu = prop.multiProcessorCount;
+    *out = ctx;
+    return MDISCR_OK;
+}
+
+mdiscr_status mdiscr_ctx_destroy(mdiscr_ctx* ctx) {
+    if (!ctx) return MDISCR_EINVAL;
+    hipSetDevice(ctx->device);
+    hipDeviceSynchronize();
+    for (auto& t : ctx->pending) {
+        hipEventDestroy(t.a);
+        hipEventDestroy(t.b);
+    }
+    if (ctx->ws) hipFree(ctx->ws);
+    ctx_release_extra(ctx);
+    delete ctx;
+    return MDISCR_OK;
+}
+
+mdiscr_status mdiscr_sync(mdiscr_ctx* ctx, void* stream) {
+    if (!ctx) return MDISCR_EINVAL;
+    return ctx_hip(ctx, hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+}
+
+const char* mdiscr_status_str(mdiscr_status s) {
+    switch (s) {
+        case MDISCR_OK: return "ok";
+        case MDISCR_EINVAL: return "invalid argument";
+        case MDISCR_ENOMEM: return "out of memory";
+        case MDISCR_EDEVICE: return "device error";
+        case MDISCR_EBUSY: return "busy";
+        case MDISCR_EUNSUPPORTED: return "unsupported";
+    }
+    return "unknown status";
+}
+
+const char* mdiscr_last_error(const mdiscr_ctx* ctx) {
+    return ctx ? ctx->err : "no context";
+}
+
+mdiscr_status mdiscr_timing_enable(mdiscr_ctx* ctx, int enable) {
+    if (!ctx) return MDISCR_EINVAL;
+    ctx->timing = enable != 0;
+    return MDISCR_OK;
+}
+
+mdiscr_status mdiscr_timing_read(mdiscr_ctx* ctx, const char* name, double* total_ms,
+                                 uint64_t* launches) {
+    if (!ctx || !name || !total_ms || !launches) return MDISCR_EINVAL;
+    // Fold every finished record into the per-name totals, then report one.
+    for (auto& t : ctx->pending) {
+        hipError_t e = hipEventSynchronize(t.b);
+        if (e != hipSuccess) return ctx_hip(ctx, e, "hipEventSynchronize");
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, t.a, t.b);
+        auto& acc = ctx->timing_acc[t.name];
+        acc.first += ms;
+        acc.second += 1;
+        hipEventDestroy(t.a);
+        hipEventDestroy(t.b);
+    }
+    ctx->pending.clear();
+    auto it = ctx->timing_acc.find(name);
+    if (it == ctx->timing_acc.end()) {
+        *total_ms = 0.0;
+        *launches = 0;
+    } else {
+        *total_ms = it->second.first;
+        *launches = it->second.second;
+        ctx->timing_acc.erase(it);
+    }
+    return MDISCR_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// plain partition
+
+namespace {
+
+constexpr int kNW = 4;    // waves per workgroup
+constexpr int kR = 16;    // rounds (elements per lane)
+constexpr int kScanThreads = 1024;
+
+template <int NBITS, class Discr>
+mdiscr_status launch_plain(mdiscr_ctx* ctx, const uint32_t* in, uint32_t* out, uint32_t n,
+                           Discr dis, uint32_t m, uint32_t* user_counts, hipStream_t s,
+                           uint32_t* ws) {
+    constexpr int TILE = TileGeom<kNW, kR>::kTile;
+    const uint32_t T = (n + TILE - 1) / TILE;
+    uint32_t* counts = ws;            // m * T
+    uint32_t* totals = ws + (size_t)m * T;
+    TimedLaunch tl;
+    timing_begin(ctx, s, tl);
+    hipLaunchKernelGGL((k_tile_hist<kNW, kR, NBITS, Discr>), dim3(T), dim3(kNW * kWave), 0, s,
+                       in, n, dis, counts, T, m);
+    timing_end(ctx, s, tl, "tile_hist");
+    timing_begin(ctx, s, tl);
+    hipLaunchKernelGGL((k_tile_scan<kScanThreads>), dim3(m), dim3(kScanThreads), 0, s, counts, T,
+                       totals, user_counts);
+    timing_end(ctx, s, tl, "tile_scan");
+    timing_begin(ctx, s, tl);
+    hipLaunchKernelGGL((k_partition<kNW, kR, NBITS, Discr>), dim3(T), dim3(kNW * kWave), 0, s,
+                       in, out, n, dis, counts, totals, T, m);
+    timing_end(ctx, s, tl, "partition");
+    return ctx_hip(ctx, hipGetLastError(), "plain partition launch");
+}
+
+template <class Discr>
+mdiscr_status dispatch_plain(mdiscr_ctx* ctx, const uint32_t* in, uint32_t* out, uint32_t n,
+                             Discr dis, uint32_t m, uint32_t* user_counts, hipStream_t s,
+                             uint32_t* ws) {
+    const uint32_t nb = class_bits(m);
+    switch (nb) {
+        case 0: return launch_plain<0>(ctx, in, out, n, dis, m, user_counts, s, ws);
+        case 1: return launch_plain<1>(ctx, in, out, n, dis, m, user_counts, s, ws);
+        case 2: return launch_plain<2>(ctx, in, out, n, dis, m, user_counts, s, ws);
+        case 3: return launch_plain<3>(ctx, in, out, n, dis, m, user_counts, s, ws);
+        case 4: return launch_plain<4>(ctx, in, out, n, dis, m, user_counts, s, ws);
+        case 5: return launch_plain<5>(ctx, in, out, n, dis, m, user_counts, s, ws);
+        case 6: return launch_plain<6>(ctx, in, out, n, dis, m, user_counts, s, ws);
+    }
+    return ctx_fail(ctx, MDISCR_EUNSUPPORTED, "m=%u", m);
+}
+
+}  // namespace
+
+mdiscr_status check_discr(mdiscr_ctx* ctx, mdiscr_discr d) {
+    if (d.m == 0 || d.m > MDISCR_MAX_CLASSES)
+        return ctx_fail(ctx, MDISCR_EINVAL, "m=%u outside [1,%u]", d.m, MDISCR_MAX_CLASSES);
+    if (d.kind == MDISCR_DISCR_MASK) {
+        if (d.m & (d.m - 1))
+            return ctx_fail(ctx, MDISCR_EINVAL, "MASK discriminator needs a power-of-two m");
+        return MDISCR_OK;
+    }
+    if (d.kind != MDISCR_DISCR_MOD)
+        return ctx_fail(ctx, MDISCR_EINVAL, "unknown discriminator kind %u", d.kind);
+    return MDISCR_OK;
+}
+
+uint32_t class_bits(uint32_t m) {
+    uint32_t b = 0;
+    while ((1u << b) < m) ++b;
+    return b;
+}
+
+extern "C" {
+
+mdiscr_status mdiscr_partition_u32(mdiscr_ctx* ctx, const uint32_t* d_in, uint32_t* d_out,
+                                   uint64_t n, mdiscr_discr d, uint32_t* d_counts,
+                                   void* stream) {
+    if (!ctx) return MDISCR_EINVAL;
+    mdiscr_status st = check_discr(ctx, d);
+    if (st) return st;
+    if (n > MDISCR_MAX_ELEMS) return ctx_fail(ctx, MDISCR_EINVAL, "n=%llu too large", (unsigned long long)n);
+    if (n && (!d_in || !d_out)) return ctx_fail(ctx, MDISCR_EINVAL, "null data pointer");
+    if (n && ranges_overlap(d_in, d_out, n * 4))
+        return ctx_fail(ctx, MDISCR_EINVAL, "output overlaps input");
+    hipStream_t s = (hipStream_t)stream;
+    st = ctx_hip(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    if (st) return st;
+    if (n == 0) {
+        if (d_counts) {
+            hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, d_counts, 0u, d.m);
+            return ctx_hip(ctx, hipGetLastError(), "fill launch");
+        }
+        return MDISCR_OK;
+    }
+    constexpr int TILE = TileGeom<kNW, kR>::kTile;
+    const uint64_t T = (n + TILE - 1) / TILE;
+    void* ws = nullptr;
+    st = ctx_workspace(ctx, (T + 1) * d.m * sizeof(uint32_t), &ws);
+    if (st) return st;
+    if (d.kind == MDISCR_DISCR_MASK || (d.m & (d.m - 1)) == 0)
+        return dispatch_plain(ctx, d_in, d_out, (uint32_t)n, DiscrMask{d.m - 1}, d.m, d_counts, s,
+                              (uint32_t*)ws);
+    return dispatch_plain(ctx, d_in, d_out, (uint32_t)n, DiscrMod{mdiscr_divmagic_make(d.m)}, d.m,
+                          d_counts, s, (uint32_t*)ws);
+}
+
+mdiscr_status mdiscr_u32(mdiscr_ctx* ctx, uint64_t n, uint32_t num_hwd_thds,
+                         const uint32_t* d_in, uint32_t* d_out, mdiscr_discr d,
+                         uint32_t* h_counts) {
+    (void)num_hwd_thds;
+    if (!ctx || !h_counts) return ctx_fail(ctx, MDISCR_EINVAL, "null argument");
+    mdiscr_status st = check_discr(ctx, d);
+    if (st) return st;
+    uint32_t* dc = nullptr;
+    st = ctx_small_buffer(ctx, d.m * sizeof(uint32_t), (void**)&dc);
+    if (st) return st;
+    st = mdiscr_partition_u32(ctx, d_in, d_out, n, d, dc, nullptr);
+    if (st) return st;
+    st = ctx_hip(ctx, hipMemcpy(h_counts, dc, d.m * sizeof(uint32_t), hipMemcpyDeviceToHost),
+                 "counts readback");
+    return st;
+}
+
+mdiscr_status mdiscr_sizes_from_counts(mdiscr_ctx* ctx, const uint32_t* d_counts, uint32_t m,
+                                       uint64_t n, uint32_t* d_sizes, void* stream) {
+    if (!ctx) return MDISCR_EINVAL;
+    if (!d_counts || m == 0 || m > MDISCR_MAX_CLASSES || (n && !d_sizes) || n > MDISCR_MAX_ELEMS)
+        return ctx_fail(ctx, MDISCR_EINVAL, "bad sizes_from_counts arguments");
+    if (n == 0) return MDISCR_OK;
+    hipStream_t s = (hipStream_t)stream;
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 255) / 256, 8192);
+    TimedLaunch tl;
+    timing_begin(ctx, s, tl);
+    hipLaunchKernelGGL(k_sizes_from_counts, dim3(blocks), dim3(256), 0, s, d_counts, m, d_sizes,
+                       (uint32_t)n);
+    timing_end(ctx, s, tl, "sizes_from_counts");
+    return ctx_hip(ctx, hipGetLastError(), "sizes launch");
+}
+
+}  // extern "C"
+
+mdiscr_status ctx_small_buffer(mdiscr_ctx* ctx, size_t bytes, void** out) {
+    constexpr size_t kSmall = 64 * 1024;
+    if (bytes > kSmall) return ctx_fail(ctx, MDISCR_EINVAL, "small buffer request too large");
+    if (!ctx->small) {
+        hipError_t e = hipMalloc(&ctx->small, kSmall);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return ctx_fail(ctx, MDISCR_ENOMEM, "small buffer: %s", hipGetErrorString(e));
+        }
+    }
+    *out = ctx->small;
+    return MDISCR_OK;
+}
