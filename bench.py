@@ -146,7 +146,7 @@ def main():
     dt = time_steps(ctx, step, args.steps, args.warmup)
     value = N * args.steps / dt / 1e9
     ms_step = dt / args.steps * 1e3
-    kernels = ["tile_hist", "tile_scan", "partition"]
+    kernels = ["chunk_count", "tile_scan", "partition"]
     kt = kernel_times(ctx, step, max(args.steps, 10), kernels)
     dtz = time_steps(ctx, step_zipf, args.steps, 2)
     ktz = kernel_times(ctx, step_zipf, max(args.steps, 10), kernels)

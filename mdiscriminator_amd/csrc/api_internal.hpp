@@ -44,3 +44,9 @@ inline bool ranges_overlap(const void* a, const void* b, uint64_t bytes) {
     const uintptr_t x = (uintptr_t)a, y = (uintptr_t)b;
     return x < y + bytes && y < x + bytes;
 }
+
+// [a, a+na) and [b, b+nb) share a byte.
+inline bool extents_overlap(const void* a, uint64_t na, const void* b, uint64_t nb) {
+    const uintptr_t x = (uintptr_t)a, y = (uintptr_t)b;
+    return na && nb && x < y + nb && y < x + na;
+}

@@ -13,6 +13,8 @@ constexpr int kWave = 64;      // CDNA wavefront width
 constexpr int kNumXcd = 8;     // MI355X: 8 XCDs, workgroups dealt round-robin
 constexpr int kMaxClasses = 64;
 
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
 // ---- discriminators (the class of a 32-bit key) ---------------------------
 
 // Mod4 / Mod8 / Mod<2^k> (HelpersCommon.h:593-595, 846-848): x & (m-1).
@@ -71,15 +73,18 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 template <int NBITS>
 __device__ __forceinline__ void wave_match_rank(uint32_t c, uint32_t lane,
                                                 uint32_t& rank, uint32_t& cnt) {
-    uint64_t mism = 0;
+    uint32_t mlo = 0, mhi = 0;
 #pragma unroll
     for (int b = 0; b < NBITS; ++b) {
-        const bool bit = (c >> b) & 1u;
-        const uint64_t B = ballot64(bit);
-        mism |= bit ? ~B : B;
+        // splat = all ones if this lane's bit b is set; mismatching lanes are
+        // the ballot bits that differ from it.
+        const uint32_t splat = (uint32_t)__builtin_amdgcn_sbfe((int)c, b, 1);
+        const uint64_t B = ballot64(splat != 0u);
+        mlo |= (uint32_t)B ^ splat;
+        mhi |= (uint32_t)(B >> 32) ^ splat;
     }
-    rank = lane - mbcnt64(mism);
-    cnt = kWave - (uint32_t)__popcll(mism);
+    rank = lane - __builtin_amdgcn_mbcnt_hi(mhi, __builtin_amdgcn_mbcnt_lo(mlo, 0u));
+    cnt = kWave - (uint32_t)(__builtin_popcount(mlo) + __builtin_popcount(mhi));
 }
 
 // ---- XCD-aware tile order -------------------------------------------------------

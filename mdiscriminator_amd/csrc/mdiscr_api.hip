@@ -176,30 +176,64 @@ mdiscr_status mdiscr_timing_read(mdiscr_ctx* ctx, const char* name, double* tota
 
 namespace {
 
-constexpr int kNW = 4;    // waves per workgroup
-constexpr int kR = 16;    // rounds (elements per lane)
+#ifndef MDISCR_PLAIN_NW
+#define MDISCR_PLAIN_NW 4
+#endif
+#ifndef MDISCR_PLAIN_R
+#define MDISCR_PLAIN_R 16
+#endif
+#ifndef MDISCR_PLAIN_CT
+#define MDISCR_PLAIN_CT 1
+#endif
+#ifndef MDISCR_COUNT_CT
+#define MDISCR_COUNT_CT 8
+#endif
+#ifndef MDISCR_PLAIN_PREFETCH
+#define MDISCR_PLAIN_PREFETCH 0
+#endif
+#ifndef MDISCR_PLAIN_WB
+#define MDISCR_PLAIN_WB 8
+#endif
+#ifndef MDISCR_PLAIN_MINW
+#define MDISCR_PLAIN_MINW 1
+#endif
+constexpr int kNW = MDISCR_PLAIN_NW;  // waves per workgroup
+constexpr int kR = MDISCR_PLAIN_R;    // rounds (elements per lane per tile)
+constexpr int kCT = MDISCR_PLAIN_CT;  // tiles per partition workgroup
+constexpr int kCCT = MDISCR_COUNT_CT; // tiles per count workgroup
+constexpr bool kPrefetch = MDISCR_PLAIN_PREFETCH != 0;
 constexpr int kScanThreads = 1024;
+constexpr int kScanPer = 64;
+using PlainGeom = Geom<kNW, kR, kCT>;
+constexpr uint32_t kTile = PlainGeom::kTile;
 
 template <int NBITS, class Discr>
 mdiscr_status launch_plain(mdiscr_ctx* ctx, const uint32_t* in, uint32_t* out, uint32_t n,
                            Discr dis, uint32_t m, uint32_t* user_counts, hipStream_t s,
                            uint32_t* ws) {
-    constexpr int TILE = TileGeom<kNW, kR>::kTile;
-    const uint32_t T = (n + TILE - 1) / TILE;
-    uint32_t* counts = ws;            // m * T
-    uint32_t* totals = ws + (size_t)m * T;
+    const uint32_t T = (uint32_t)(((uint64_t)n + kTile - 1) / kTile);
+    const uint32_t ld = (T + 3u) & ~3u;     // 16-byte aligned class rows
+    const uint32_t Kc = (T + kCCT - 1) / kCCT;
+    const uint32_t Kp = (T + kCT - 1) / kCT;
+    uint32_t* counts = ws;                  // m * ld
+    uint32_t* totals = ws + (size_t)m * ld;
+    const bool vec = ((uintptr_t)in & 15u) == 0;
     TimedLaunch tl;
     timing_begin(ctx, s, tl);
-    hipLaunchKernelGGL((k_tile_hist<kNW, kR, NBITS, Discr>), dim3(T), dim3(kNW * kWave), 0, s,
-                       in, n, dis, counts, T, m);
-    timing_end(ctx, s, tl, "tile_hist");
+    if (vec)
+        hipLaunchKernelGGL((k_chunk_count<kNW, kR, kCCT, NBITS, true, Discr>), dim3(Kc),
+                           dim3(kNW * kWave), 0, s, in, n, dis, counts, ld, m);
+    else
+        hipLaunchKernelGGL((k_chunk_count<kNW, kR, kCCT, NBITS, false, Discr>), dim3(Kc),
+                           dim3(kNW * kWave), 0, s, in, n, dis, counts, ld, m);
+    timing_end(ctx, s, tl, "chunk_count");
     timing_begin(ctx, s, tl);
-    hipLaunchKernelGGL((k_tile_scan<kScanThreads>), dim3(m), dim3(kScanThreads), 0, s, counts, T,
-                       totals, user_counts);
+    hipLaunchKernelGGL((k_tile_scan<kScanThreads, kScanPer>), dim3(m), dim3(kScanThreads), 0, s,
+                       counts, T, ld, totals, user_counts);
     timing_end(ctx, s, tl, "tile_scan");
     timing_begin(ctx, s, tl);
-    hipLaunchKernelGGL((k_partition<kNW, kR, NBITS, Discr>), dim3(T), dim3(kNW * kWave), 0, s,
-                       in, out, n, dis, counts, totals, T, m);
+    hipLaunchKernelGGL((k_partition<kNW, kR, kCT, NBITS, kPrefetch, MDISCR_PLAIN_WB, MDISCR_PLAIN_MINW, Discr>), dim3(Kp),
+                       dim3(kNW * kWave), 0, s, in, out, n, dis, counts, totals, ld, m);
     timing_end(ctx, s, tl, "partition");
     return ctx_hip(ctx, hipGetLastError(), "plain partition launch");
 }
@@ -259,15 +293,15 @@ mdiscr_status mdiscr_partition_u32(mdiscr_ctx* ctx, const uint32_t* d_in, uint32
     if (st) return st;
     if (n == 0) {
         if (d_counts) {
-            hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, d_counts, 0u, d.m);
+            hipLaunchKernelGGL(k_fill_u32<>, dim3(1), dim3(64), 0, s, d_counts, 0u, d.m);
             return ctx_hip(ctx, hipGetLastError(), "fill launch");
         }
         return MDISCR_OK;
     }
-    constexpr int TILE = TileGeom<kNW, kR>::kTile;
-    const uint64_t T = (n + TILE - 1) / TILE;
+    const uint64_t T = (n + kTile - 1) / kTile;
+    const uint64_t ld = (T + 3) & ~3ull;
     void* ws = nullptr;
-    st = ctx_workspace(ctx, (T + 1) * d.m * sizeof(uint32_t), &ws);
+    st = ctx_workspace(ctx, (ld + 1) * d.m * sizeof(uint32_t), &ws);
     if (st) return st;
     if (d.kind == MDISCR_DISCR_MASK || (d.m & (d.m - 1)) == 0)
         return dispatch_plain(ctx, d_in, d_out, (uint32_t)n, DiscrMask{d.m - 1}, d.m, d_counts, s,
@@ -303,7 +337,7 @@ mdiscr_status mdiscr_sizes_from_counts(mdiscr_ctx* ctx, const uint32_t* d_counts
     const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 255) / 256, 8192);
     TimedLaunch tl;
     timing_begin(ctx, s, tl);
-    hipLaunchKernelGGL(k_sizes_from_counts, dim3(blocks), dim3(256), 0, s, d_counts, m, d_sizes,
+    hipLaunchKernelGGL(k_sizes_from_counts<>, dim3(blocks), dim3(256), 0, s, d_counts, m, d_sizes,
                        (uint32_t)n);
     timing_end(ctx, s, tl, "sizes_from_counts");
     return ctx_hip(ctx, hipGetLastError(), "sizes launch");

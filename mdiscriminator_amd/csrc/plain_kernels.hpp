@@ -4,19 +4,22 @@
 // pipeline (MdiscrHost_optimized_clean.cu.h:47-118,
 // MdiscrKernels_optimized_clean.cu.h:10-183) with three launches:
 //
-//   tile_hist   reads each tile once and writes its m class counts
-//               (class-major, counts[c*T + t]).                      4 B/elem
+//   chunk_count reads CT tiles per workgroup once and writes every tile's m
+//               class counts (class-major, counts[c*ld + t]).        4 B/elem
 //   tile_scan   one workgroup per class: exclusive scan of that class's
 //               counts over the T tiles, in place; writes the class total.
-//   partition   reads each tile once, ranks every element inside its wave with
+//   partition   CT tiles per workgroup in order (one, as launched), carrying
+//               each class's running global offset; reads each tile once,
+//               ranks every element inside its wave with
 //               NBITS ballots (stable: lower lanes and earlier rounds first),
 //               scans the per-wave class counts into per-class LDS cursors,
 //               stages the tile in LDS grouped by class, then writes each
 //               class run contiguously to
-//                   base[c] + prefix[c][t] + (slot - tile_start[c]).  8 B/elem
+//                   running[c] + (slot - tile_start[c]).               8 B/elem
 //
-// The class of the output rows is fixed by the global totals, which are known
-// only after every element has been read once; tile_hist is that read.
+// Where each class starts in the output is fixed by the global totals, which
+// are known only after every element has been read once; chunk_count is that
+// read.
 //
 // Tile geometry: NW waves x 64 lanes x R rounds, round r of wave w holding
 // elements tile_base + w*64*R + r*64 + lane, so a wave's element order is
@@ -28,212 +31,389 @@
 
 namespace mdiscr {
 
-template <int NW, int R>
-struct TileGeom {
+template <int NW, int R, int CT>
+struct Geom {
     static constexpr int kThreads = NW * kWave;
-    static constexpr int kTile = NW * kWave * R;
+    static constexpr int kTile = NW * kWave * R;   // elements per tile
+    static constexpr int kChunkTiles = CT;           // tiles per workgroup
+    static constexpr int kChunk = kTile * CT;        // elements per workgroup
 };
 
 // ---------------------------------------------------------------------------
-// tile_hist: counts[c*T + t] = number of class-c elements of tile t.
-template <int NW, int R, int NBITS, class Discr>
-__global__ __launch_bounds__(NW* kWave) void k_tile_hist(
+// Per-lane class counters for the count pass.  NBITS <= 4: byte counters
+// packed in one or two 64-bit words (one shift and a 64-bit add per element);
+// wider class sets count with LDS atomics.
+template <int NBITS>
+struct LaneCounts {
+    static constexpr bool kPacked = NBITS <= 4;
+    uint64_t lo = 0, hi = 0;
+    __device__ __forceinline__ void add(uint32_t c) {
+        const uint64_t inc = 1ull << ((c & 7u) * 8u);
+        if (NBITS <= 3) {
+            lo += inc;
+        } else {
+            const bool up = (c & 8u) != 0;
+            lo += up ? 0ull : inc;
+            hi += up ? inc : 0ull;
+        }
+    }
+    // Sum over the wave; lane c (< 2^NBITS) receives the count of class c.
+    __device__ __forceinline__ uint32_t wave_class_count(uint32_t lane) const {
+        constexpr uint64_t kEven = 0x00FF00FF00FF00FFull;
+        uint32_t e0 = (uint32_t)(lo & kEven), e1 = (uint32_t)((lo & kEven) >> 32);
+        uint32_t o0 = (uint32_t)((lo >> 8) & kEven), o1 = (uint32_t)(((lo >> 8) & kEven) >> 32);
+        e0 = wave_sum(e0); e1 = wave_sum(e1); o0 = wave_sum(o0); o1 = wave_sum(o1);
+        uint32_t h0 = 0, h1 = 0, g0 = 0, g1 = 0;
+        if (NBITS == 4) {
+            h0 = wave_sum((uint32_t)(hi & kEven));
+            h1 = wave_sum((uint32_t)((hi & kEven) >> 32));
+            g0 = wave_sum((uint32_t)((hi >> 8) & kEven));
+            g1 = wave_sum((uint32_t)(((hi >> 8) & kEven) >> 32));
+        }
+        // class c = byte c: even bytes in e/h, odd bytes in o/g; 16-bit fields.
+        const uint32_t c = lane & 15u;
+        const bool up = c >= 8u;
+        const uint32_t b = c & 7u;
+        const uint32_t w0 = (b & 1u) ? (up ? g0 : o0) : (up ? h0 : e0);
+        const uint32_t w1 = (b & 1u) ? (up ? g1 : o1) : (up ? h1 : e1);
+        const uint32_t f = b >> 1;  // field 0..3 across (w0, w1)
+        const uint32_t w = (f < 2u) ? w0 : w1;
+        return (w >> ((f & 1u) * 16u)) & 0xFFFFu;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// count pass: each workgroup reads CT consecutive tiles and writes every
+// tile's class counts, counts[c*ld + t] (class-major rows of ld >= T entries).
+// VEC: the input is 16-byte aligned, so full tiles use 16-byte loads (a count
+// does not depend on element order).
+template <int NW, int R, int CT, int NBITS, bool VEC, class Discr>
+__global__ __launch_bounds__(NW* kWave) void k_chunk_count(
     const uint32_t* __restrict__ in, uint32_t n, Discr dis,
-    uint32_t* __restrict__ counts, uint32_t T, uint32_t m) {
+    uint32_t* __restrict__ counts, uint32_t ld, uint32_t m) {
+    using G = Geom<NW, R, CT>;
     constexpr int MC = 1 << NBITS;
-    constexpr int TILE = TileGeom<NW, R>::kTile;
-    __shared__ uint32_t hist[NW][MC];
+    constexpr int TILE = G::kTile;
+    constexpr int THREADS = G::kThreads;
+    constexpr bool PACKED = LaneCounts<NBITS>::kPacked;
+    static_assert(R <= 255, "byte counters per lane");
+    static_assert(R % 4 == 0, "16-byte loads");
+    __shared__ uint32_t hist[2][NW][MC];  // double-buffered across tiles
 
     const uint32_t lane = lane_id();
     const uint32_t wave = threadIdx.x / kWave;
-    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
-    const uint32_t tbase = tile * (uint32_t)TILE;
-    const uint32_t wbase = tbase + wave * (uint32_t)(kWave * R);
+    const uint32_t chunk = xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t T = (n + TILE - 1) / TILE;
+    const uint32_t t0 = chunk * CT;
+    const uint32_t t1 = min(t0 + (uint32_t)CT, T);
 
-    for (int c = lane; c < MC; c += kWave) hist[wave][c] = 0;
-    __builtin_amdgcn_wave_barrier();
-
-    uint32_t v[R];
-    const bool full = (uint64_t)tbase + TILE <= n;
-    if (full) {
+    for (uint32_t t = t0; t < t1; ++t) {
+        uint32_t (*h)[MC] = hist[(t - t0) & 1u];
+        if (!PACKED) {
+            for (int c = lane; c < MC; c += kWave) h[wave][c] = 0;
+            __builtin_amdgcn_wave_barrier();
+        }
+        LaneCounts<NBITS> lc;
+        const uint32_t tbase = t * (uint32_t)TILE;
+        const bool full = (uint64_t)tbase + TILE <= n;
+        if (VEC && full) {
+            const u32x4* in4 = reinterpret_cast<const u32x4*>(in + tbase);
+            u32x4 q[R / 4];
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[r] = __builtin_nontemporal_load(in + wbase + r * kWave + lane);
-    } else {
+            for (int k = 0; k < R / 4; ++k) q[k] = __builtin_nontemporal_load(in4 + k * THREADS + threadIdx.x);
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const uint32_t idx = wbase + r * kWave + lane;
-            v[r] = idx < n ? in[idx] : 0u;
+            for (int k = 0; k < R / 4; ++k) {
+                const uint32_t e[4] = {q[k].x, q[k].y, q[k].z, q[k].w};
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const uint32_t c = dis(e[j]);
+                    if (PACKED) lc.add(c);
+                    else atomicAdd(&h[wave][c], 1u);
+                }
+            }
+        } else {
+            uint32_t v[R];
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                const uint32_t idx = tbase + k * THREADS + threadIdx.x;
+                v[k] = (full || idx < n) ? in[idx] : 0u;
+            }
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                const uint32_t idx = tbase + k * THREADS + threadIdx.x;
+                if (full || idx < n) {
+                    const uint32_t c = dis(v[k]);
+                    if (PACKED) lc.add(c);
+                    else atomicAdd(&h[wave][c], 1u);
+                }
+            }
+        }
+        if (PACKED) {
+            const uint32_t cnt = lc.wave_class_count(lane);
+            if (lane < (uint32_t)MC) h[wave][lane] = cnt;
+        }
+        __syncthreads();
+        for (uint32_t c = threadIdx.x; c < m; c += THREADS) {
+            uint32_t s = 0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) s += h[w][c];
+            counts[(size_t)c * ld + t] = s;
         }
     }
-    __builtin_amdgcn_wave_barrier();
-    uint32_t mine = 0;  // NBITS <= 3: lane k accumulates class k's count
+}
+
+// ---------------------------------------------------------------------------
+// tile scan: for class c = blockIdx.x, row = counts + c*ld (ld a multiple of
+// 4, so rows are 16-byte aligned): row[t] <- sum_{t'<t} row[t'] for t < T,
+// totals[c] <- the class total (and user_counts[c] when given).  Each thread
+// owns PER consecutive entries per pass, fetched with PER/4 16-byte loads.
+template <int THREADS, int PER>
+__global__ __launch_bounds__(THREADS) void k_tile_scan(
+    uint32_t* __restrict__ counts, uint32_t T, uint32_t ld, uint32_t* __restrict__ totals,
+    uint32_t* __restrict__ user_counts) {
+    static_assert(PER % 4 == 0, "vector loads");
+    constexpr int NWAVE = THREADS / kWave;
+    __shared__ uint32_t wsum[NWAVE];
+    uint32_t* row = counts + (size_t)blockIdx.x * ld;
+    const uint32_t lane = lane_id(), wave = threadIdx.x / kWave;
+    uint32_t carry = 0;
+    for (uint32_t pbase = 0; pbase < T; pbase += THREADS * PER) {
+        const uint32_t beg = pbase + threadIdx.x * PER;
+        uint32_t x[PER];
+        if (beg + PER <= T) {
+#pragma unroll
+            for (int j = 0; j < PER / 4; ++j) {
+                const u32x4 q = reinterpret_cast<const u32x4*>(row + beg)[j];
+                x[4 * j] = q.x; x[4 * j + 1] = q.y; x[4 * j + 2] = q.z; x[4 * j + 3] = q.w;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < PER; ++j) x[j] = (beg + j < T) ? row[beg + j] : 0u;
+        }
+        uint32_t s = 0;
+#pragma unroll
+        for (int j = 0; j < PER; ++j) s += x[j];
+        const uint32_t wex = wave_excl_scan(s, lane);
+        if (lane == kWave - 1) wsum[wave] = wex + s;
+        __syncthreads();
+        uint32_t wpre = 0, tot = 0;
+#pragma unroll
+        for (int w = 0; w < NWAVE; ++w) {
+            wpre += (w < (int)wave) ? wsum[w] : 0u;
+            tot += wsum[w];
+        }
+        uint32_t acc = carry + wpre + wex;
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const uint32_t e = x[j];
+            x[j] = acc;
+            acc += e;
+        }
+        if (beg + PER <= T) {
+#pragma unroll
+            for (int j = 0; j < PER / 4; ++j) {
+                u32x4 q;
+                q.x = x[4 * j]; q.y = x[4 * j + 1]; q.z = x[4 * j + 2]; q.w = x[4 * j + 3];
+                reinterpret_cast<u32x4*>(row + beg)[j] = q;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < PER; ++j)
+                if (beg + j < T) row[beg + j] = x[j];
+        }
+        carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        totals[blockIdx.x] = carry;
+        if (user_counts) user_counts[blockIdx.x] = carry;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Tile phases shared by the plain and struct-of-arrays partitions.
+
+// Phase 1: in-wave stable rank of R rounds of 64 keys.  Round r of the wave
+// holds elements wbase + r*64 + lane, so (round, lane) is memory order.  When
+// !FULL, elements past n take class m-1: they sit at the end of the last tile,
+// so they rank after every real class m-1 element and land in LDS slots
+// [valid, TILE), which are never written out.  packed[r] = class<<16 | rank
+// within (wave, class); cursor[c] ends as this wave's class-c count.
+template <int R, int NBITS, bool FULL, class Discr>
+__device__ __forceinline__ void rank_rounds(const uint32_t (&v)[R], uint32_t (&packed)[R],
+                                            uint32_t* cursor, uint32_t wbase, uint32_t n,
+                                            uint32_t m, uint32_t lane, Discr dis) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        uint32_t c = dis(v[r]);
+        if (!FULL) c = (wbase + r * kWave + lane < n) ? c : m - 1u;
+        uint32_t rk, cnt;
+        wave_match_rank<NBITS>(c, lane, rk, cnt);
+        const uint32_t run = cursor[c];
+        if (rk == cnt - 1u) cursor[c] = run + cnt;
+        packed[r] = (c << 16) | (run + rk);
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// Phase 2 (wave 0, lane = class): turn the per-wave class counts in
+// cursor[w][c] into LDS slot bases (class-major, wave-minor), set
+// dpos[c] = running[c] - tile_start[c], and advance running[c].
+template <int NW, int MC>
+__device__ __forceinline__ void tile_offsets(uint32_t (*cursor)[MC], uint32_t* dpos,
+                                             uint32_t& running, uint32_t lane) {
+    uint32_t tot = 0;
+    if (lane < (uint32_t)MC) {
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            const uint32_t x = cursor[w][lane];
+            cursor[w][lane] = tot;
+            tot += x;
+        }
+    }
+    const uint32_t toff = wave_excl_scan(tot, lane);
+    if (lane < (uint32_t)MC) {
+#pragma unroll
+        for (int w = 0; w < NW; ++w) cursor[w][lane] += toff;
+        dpos[lane] = running - toff;
+    }
+    running += tot;
+}
+
+template <int R, bool FULL>
+__device__ __forceinline__ void load_rounds(const uint32_t* __restrict__ in, uint32_t (&v)[R],
+                                            uint32_t wbase, uint32_t n, uint32_t lane) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const uint32_t idx = wbase + r * kWave + lane;
-        const bool ok = full || idx < n;
-        const uint32_t c = dis(v[r]);
-        if (NBITS <= 3) {
-            // Few classes: one ballot per class, counted on the scalar unit.
+        if (FULL) v[r] = __builtin_nontemporal_load(in + idx);
+        else v[r] = idx < n ? __builtin_nontemporal_load(in + idx) : 0u;
+    }
+}
+
+// Phase 4: slot s = k*THREADS + tid goes to out[dpos[class] + s], in batches
+// of WB slots per thread (the LDS reads of a batch are issued together).
+template <int R, int THREADS, int WB, bool FULL, class Discr>
+__device__ __forceinline__ void write_runs(const uint32_t* stage, const uint32_t* dpos,
+                                           uint32_t* __restrict__ out, uint32_t valid,
+                                           Discr dis) {
+    static_assert(R % WB == 0, "write batch");
 #pragma unroll
-            for (int k = 0; k < MC; ++k) {
-                const uint64_t B = ballot64(ok && c == (uint32_t)k);
-                if ((uint32_t)k == lane) mine += (uint32_t)__popcll(B);
-            }
-        } else {
-            if (ok) atomicAdd(&hist[wave][c], 1u);
+    for (int k0 = 0; k0 < R; k0 += WB) {
+        uint32_t x[WB], d[WB];
+#pragma unroll
+        for (int k = 0; k < WB; ++k) {
+            const uint32_t s = (k0 + k) * THREADS + threadIdx.x;
+            x[k] = (FULL || s < valid) ? stage[s] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < WB; ++k) d[k] = dpos[dis(x[k])];
+#pragma unroll
+        for (int k = 0; k < WB; ++k) {
+            const uint32_t s = (k0 + k) * THREADS + threadIdx.x;
+            if (FULL || s < valid) __builtin_nontemporal_store(x[k], out + (d[k] + s));
         }
     }
-    if (NBITS <= 3 && lane < (uint32_t)MC) hist[wave][lane] = mine;
-    __syncthreads();
-    for (uint32_t c = threadIdx.x; c < m; c += NW * kWave) {
-        uint32_t s = 0;
-#pragma unroll
-        for (int w = 0; w < NW; ++w) s += hist[w][c];
-        counts[(size_t)c * T + tile] = s;
-    }
 }
 
 // ---------------------------------------------------------------------------
-// tile_scan: for class c = blockIdx.x, counts[c*T + t] <- sum_{t'<t} counts,
-// totals[c] <- sum over all tiles; user_counts[c] too when given.
-template <int THREADS>
-__global__ __launch_bounds__(THREADS) void k_tile_scan(
-    uint32_t* __restrict__ counts, uint32_t T, uint32_t* __restrict__ totals,
-    uint32_t* __restrict__ user_counts) {
-    constexpr int NWAVE = THREADS / kWave;
-    __shared__ uint32_t wsum[NWAVE];
-    uint32_t* row = counts + (size_t)blockIdx.x * T;
-    const uint32_t chunk = (T + THREADS - 1) / THREADS;
-    const uint32_t beg = threadIdx.x * chunk;
-    const uint32_t end = min(beg + chunk, T);
-    uint32_t s = 0;
-    for (uint32_t t = beg; t < end; ++t) s += row[t];
-    const uint32_t lane = lane_id(), wave = threadIdx.x / kWave;
-    const uint32_t wex = wave_excl_scan(s, lane);
-    if (lane == kWave - 1) wsum[wave] = wex + s;
-    __syncthreads();
-    uint32_t wpre = 0, tot = 0;
-#pragma unroll
-    for (int w = 0; w < NWAVE; ++w) {
-        wpre += (w < (int)wave) ? wsum[w] : 0u;
-        tot += wsum[w];
-    }
-    uint32_t acc = wpre + wex;
-    for (uint32_t t = beg; t < end; ++t) {
-        const uint32_t x = row[t];
-        row[t] = acc;
-        acc += x;
-    }
-    if (threadIdx.x == 0) {
-        totals[blockIdx.x] = tot;
-        if (user_counts) user_counts[blockIdx.x] = tot;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// partition: stable scatter of one tile per workgroup.
-template <int NW, int R, int NBITS, class Discr>
-__global__ __launch_bounds__(NW* kWave) void k_partition(
+// partition: each workgroup runs its chunk's tiles in order, carrying the
+// running global offset of every class across tiles (wave 0, lane c).
+template <int NW, int R, int CT, int NBITS, bool PREFETCH, int WB, int MINW, class Discr>
+__global__ __launch_bounds__(NW* kWave, MINW) void k_partition(
     const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t n,
     Discr dis, const uint32_t* __restrict__ prefix,
-    const uint32_t* __restrict__ totals, uint32_t T, uint32_t m) {
+    const uint32_t* __restrict__ totals, uint32_t ld, uint32_t m) {
+    using G = Geom<NW, R, CT>;
     constexpr int MC = 1 << NBITS;
-    constexpr int TILE = TileGeom<NW, R>::kTile;
-    constexpr int THREADS = NW * kWave;
+    constexpr int TILE = G::kTile;
+    constexpr int THREADS = G::kThreads;
     __shared__ uint32_t stage[TILE];
     __shared__ uint32_t cursor[NW][MC];  // per-wave running count, then LDS base
     __shared__ uint32_t dpos[MC];        // global destination - LDS slot, per class
 
     const uint32_t lane = lane_id();
     const uint32_t wave = threadIdx.x / kWave;
-    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
-    const uint32_t tbase = tile * (uint32_t)TILE;
-    const uint32_t wbase = tbase + wave * (uint32_t)(kWave * R);
-    const bool full = (uint64_t)tbase + TILE <= n;
-    const uint32_t valid = full ? (uint32_t)TILE : n - tbase;
+    const uint32_t chunk = xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t T = (n + TILE - 1) / TILE;
+    const uint32_t t0 = chunk * CT;
+    const uint32_t t1 = min(t0 + (uint32_t)CT, T);
 
-    for (int c = lane; c < MC; c += kWave) cursor[wave][c] = 0;
-    __builtin_amdgcn_wave_barrier();
-
-    // Wave 0 fetches this tile's class prefixes and the class totals early.
-    uint32_t my_prefix = 0, my_total = 0;
+    // Wave 0, lane c: the first tile's class-c prefix and the class-c total;
+    // the running position of the next class-c element is formed at the first
+    // tile's phase 2, after these loads have landed.
+    uint32_t pre = 0, tot = 0;
     if (wave == 0 && lane < m) {
-        my_prefix = prefix[(size_t)lane * T + tile];
-        my_total = totals[lane];
+        pre = prefix[(size_t)lane * ld + t0];
+        tot = totals[lane];
     }
 
     uint32_t v[R];
-    if (full) {
-#pragma unroll
-        for (int r = 0; r < R; ++r) v[r] = __builtin_nontemporal_load(in + wbase + r * kWave + lane);
-    } else {
+    {
+        const uint32_t wbase = t0 * (uint32_t)TILE + wave * (uint32_t)(kWave * R);
+        if ((uint64_t)(t0 + 1) * TILE <= n) load_rounds<R, true>(in, v, wbase, n, lane);
+        else load_rounds<R, false>(in, v, wbase, n, lane);
+    }
+
+    uint32_t running = 0;
+    for (uint32_t t = t0; t < t1; ++t) {
+        const uint32_t tbase = t * (uint32_t)TILE;
+        const uint32_t wbase = tbase + wave * (uint32_t)(kWave * R);
+        const bool full = (uint64_t)tbase + TILE <= n;
+        const uint32_t valid = full ? (uint32_t)TILE : n - tbase;
+
+        for (int c = lane; c < MC; c += kWave) cursor[wave][c] = 0;
+        __builtin_amdgcn_wave_barrier();
+
+        // 1. In-wave stable rank.
+        uint32_t packed[R];
+        if (full) rank_rounds<R, NBITS, true>(v, packed, cursor[wave], wbase, n, m, lane, dis);
+        else rank_rounds<R, NBITS, false>(v, packed, cursor[wave], wbase, n, m, lane, dis);
+        __syncthreads();
+
+        // Prefetch the next tile while this one is placed and written.
+        uint32_t vn[R];
+        if (PREFETCH && t + 1 < t1) {
+            const uint32_t nb = wbase + (uint32_t)TILE;
+            if ((uint64_t)(tbase + TILE) + TILE <= n) load_rounds<R, true>(in, vn, nb, n, lane);
+            else load_rounds<R, false>(in, vn, nb, n, lane);
+        }
+
+        // 2. Per-class LDS cursors and global offsets.
+        if (wave == 0) {
+            if (t == t0) running = wave_excl_scan(tot, lane) + pre;
+            tile_offsets<NW, MC>(cursor, dpos, running, lane);
+        }
+        __syncthreads();
+
+        // 3. Stage the tile in LDS grouped by class.
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const uint32_t idx = wbase + r * kWave + lane;
-            v[r] = idx < n ? in[idx] : 0u;
+            const uint32_t c = packed[r] >> 16;
+            stage[cursor[wave][c] + (packed[r] & 0xFFFFu)] = v[r];
         }
-    }
+        __syncthreads();
 
-    // 1. In-wave stable rank.  Elements past n take class m-1: they sit at the
-    //    end of the last tile, so they rank after every real class m-1 element
-    //    and land in LDS slots [valid, TILE), which are never written out.
-    uint32_t packed[R];  // class << 16 | rank within (wave, class)
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const uint32_t idx = wbase + r * kWave + lane;
-        const uint32_t c = (full || idx < n) ? dis(v[r]) : m - 1u;
-        uint32_t rk, cnt;
-        wave_match_rank<NBITS>(c, lane, rk, cnt);
-        const uint32_t run = cursor[wave][c];
-        if (rk == cnt - 1u) cursor[wave][c] = run + cnt;
-        packed[r] = (c << 16) | (run + rk);
-        __builtin_amdgcn_wave_barrier();
-    }
-    __syncthreads();
+        // 4. Contiguous class runs to global memory.
+        if (full) write_runs<R, THREADS, WB, true>(stage, dpos, out, valid, dis);
+        else write_runs<R, THREADS, WB, false>(stage, dpos, out, valid, dis);
 
-    // 2. Per-class LDS cursors (class-major, wave-minor) and global offsets.
-    if (wave == 0) {
-        uint32_t tot = 0;
-        if (lane < (uint32_t)MC) {
+        if (t + 1 < t1) {
+            if (PREFETCH) {
 #pragma unroll
-            for (int w = 0; w < NW; ++w) {
-                const uint32_t x = cursor[w][lane];
-                cursor[w][lane] = tot;
-                tot += x;
+                for (int r = 0; r < R; ++r) v[r] = vn[r];
+            } else {
+                const uint32_t nb = wbase + (uint32_t)TILE;
+                if ((uint64_t)(tbase + TILE) + TILE <= n) load_rounds<R, true>(in, v, nb, n, lane);
+                else load_rounds<R, false>(in, v, nb, n, lane);
             }
-        }
-        const uint32_t toff = wave_excl_scan(tot, lane);
-        const uint32_t cbase = wave_excl_scan(my_total, lane);
-        if (lane < (uint32_t)MC) {
-#pragma unroll
-            for (int w = 0; w < NW; ++w) cursor[w][lane] += toff;
-            dpos[lane] = cbase + my_prefix - toff;
-        }
-    }
-    __syncthreads();
-
-    // 3. Stage the tile in LDS grouped by class.
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const uint32_t c = packed[r] >> 16;
-        stage[cursor[wave][c] + (packed[r] & 0xFFFFu)] = v[r];
-    }
-    __syncthreads();
-
-    // 4. Contiguous class runs to global memory.
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-        const uint32_t s = k * THREADS + threadIdx.x;
-        if (s < valid) {
-            const uint32_t x = stage[s];
-            const uint32_t c = dis(x);
-            __builtin_nontemporal_store(x, out + (dpos[c] + s));
         }
     }
 }
 
 // d_counts[c] = 0 for n == 0 (a fill kernel rather than a memset, so callers
 // may capture the call in a graph).
+template <int = 0>
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, uint32_t cnt) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < cnt) p[i] = v;
@@ -241,6 +421,7 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, uint32_t cnt) {
 
 // sizes[0,n) = 0 except sizes[off[k]] = counts[k] for non-empty classes k
 // (tupleToSizeArray, HelpersCommon.h:295-311).
+template <int = 0>
 __global__ void k_sizes_from_counts(const uint32_t* __restrict__ counts,
                                     uint32_t m, uint32_t* __restrict__ sizes,
                                     uint32_t n) {

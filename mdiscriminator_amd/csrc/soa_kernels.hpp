@@ -1,0 +1,175 @@
+// soa_kernels.hpp -- struct-of-arrays m-way stable partition on gfx950.
+//
+// Replaces the reference's three-column tuple path (MdiscrHost_tuples.cu.h:
+// 262-429, MdiscrKernels_tuples.cu.h:4-236): the class comes from one 32-bit
+// key column (ModTuple<N>::apply, MdiscrHost_tuples.cu.h:44-45) and every
+// column moves to the same destination, as raw bits.
+//
+// Same three launches as the plain path; the count pass is the plain
+// k_chunk_count over the key column.  The partition ranks the key exactly as
+// the plain kernel does, records each element's class in LDS beside its slot,
+// then moves the columns one at a time through one LDS staging buffer:
+// load (same indices as the keys) -> stage at the element's slot -> write
+// contiguous class runs.  Bytes per record: 4 (count) + 2 * sum(col bytes).
+#pragma once
+
+#include "plain_kernels.hpp"
+
+namespace mdiscr {
+
+constexpr int kMaxCols = 8;
+
+struct SoaCols {
+    const void* in[kMaxCols];
+    void* out[kMaxCols];
+    uint32_t bytes[kMaxCols];
+    uint32_t ncols;
+    uint32_t key;
+};
+
+// Moves one column of the current tile: load at the tile's element indices,
+// stage at each element's LDS slot, write slot s to out[dest[s]].
+template <typename T, int NW, int R>
+__device__ __forceinline__ void soa_move_column(const void* __restrict__ src_v,
+                                                void* __restrict__ dst_v, uint32_t wbase,
+                                                uint32_t n, bool full, const uint32_t (&slot)[R],
+                                                const uint32_t (&dest)[R], uint32_t valid,
+                                                void* stage_v, uint32_t lane) {
+    constexpr int THREADS = NW * kWave;
+    const T* __restrict__ src = static_cast<const T*>(src_v);
+    T* __restrict__ dst = static_cast<T*>(dst_v);
+    T* stage = static_cast<T*>(stage_v);
+    T x[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const uint32_t idx = wbase + r * kWave + lane;
+        x[r] = (full || idx < n) ? __builtin_nontemporal_load(src + idx) : T(0);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) stage[slot[r]] = x[r];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const uint32_t s = k * THREADS + threadIdx.x;
+        if (s < valid) __builtin_nontemporal_store(stage[s], dst + dest[k]);
+    }
+    __syncthreads();
+}
+
+// The key column is already in registers: stage and write it without a reload.
+template <int NW, int R>
+__device__ __forceinline__ void soa_write_keys(const uint32_t (&v)[R], void* __restrict__ dst_v,
+                                               const uint32_t (&slot)[R],
+                                               const uint32_t (&dest)[R], uint32_t valid,
+                                               uint32_t* stage) {
+    constexpr int THREADS = NW * kWave;
+    uint32_t* __restrict__ dst = static_cast<uint32_t*>(dst_v);
+#pragma unroll
+    for (int r = 0; r < R; ++r) stage[slot[r]] = v[r];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const uint32_t s = k * THREADS + threadIdx.x;
+        if (s < valid) __builtin_nontemporal_store(stage[s], dst + dest[k]);
+    }
+    __syncthreads();
+}
+
+template <int NW, int R, int CT, int NBITS, class Discr>
+__global__ __launch_bounds__(NW* kWave) void k_partition_soa(
+    SoaCols cols, uint32_t n, Discr dis, const uint32_t* __restrict__ prefix,
+    const uint32_t* __restrict__ totals, uint32_t ld, uint32_t m) {
+    using G = Geom<NW, R, CT>;
+    constexpr int MC = 1 << NBITS;
+    constexpr int TILE = G::kTile;
+    constexpr int THREADS = G::kThreads;
+    __shared__ uint64_t stage[TILE];     // widest column element
+    __shared__ uint8_t cls[TILE];        // class of each LDS slot
+    __shared__ uint32_t cursor[NW][MC];
+    __shared__ uint32_t dpos[MC];
+
+    const uint32_t* __restrict__ key = static_cast<const uint32_t*>(cols.in[cols.key]);
+    const uint32_t lane = lane_id();
+    const uint32_t wave = threadIdx.x / kWave;
+    const uint32_t chunk = xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t T = (n + TILE - 1) / TILE;
+    const uint32_t t0 = chunk * CT;
+    const uint32_t t1 = min(t0 + (uint32_t)CT, T);
+
+    uint32_t pre = 0, tot = 0;
+    if (wave == 0 && lane < m) {
+        pre = prefix[(size_t)lane * ld + t0];
+        tot = totals[lane];
+    }
+
+    uint32_t running = 0;
+    for (uint32_t t = t0; t < t1; ++t) {
+        const uint32_t tbase = t * (uint32_t)TILE;
+        const uint32_t wbase = tbase + wave * (uint32_t)(kWave * R);
+        const bool full = (uint64_t)tbase + TILE <= n;
+        const uint32_t valid = full ? (uint32_t)TILE : n - tbase;
+
+        uint32_t v[R];
+        if (full) load_rounds<R, true>(key, v, wbase, n, lane);
+        else load_rounds<R, false>(key, v, wbase, n, lane);
+        for (int c = lane; c < MC; c += kWave) cursor[wave][c] = 0;
+        __builtin_amdgcn_wave_barrier();
+
+        // 1. In-wave stable rank of the keys (as the plain kernel).
+        uint32_t slot[R];  // class << 16 | rank, then the LDS slot
+        if (full) rank_rounds<R, NBITS, true>(v, slot, cursor[wave], wbase, n, m, lane, dis);
+        else rank_rounds<R, NBITS, false>(v, slot, cursor[wave], wbase, n, m, lane, dis);
+        __syncthreads();
+
+        // 2. LDS cursors and global offsets (wave 0, lane = class).
+        if (wave == 0) {
+            if (t == t0) running = wave_excl_scan(tot, lane) + pre;
+            tile_offsets<NW, MC>(cursor, dpos, running, lane);
+        }
+        __syncthreads();
+
+        // 3. Slots, the class of every slot, and each output slot's destination.
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const uint32_t c = slot[r] >> 16;
+            slot[r] = cursor[wave][c] + (slot[r] & 0xFFFFu);
+            cls[slot[r]] = (uint8_t)c;
+        }
+        __syncthreads();
+        uint32_t dest[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const uint32_t s = k * THREADS + threadIdx.x;
+            dest[k] = (full || s < valid) ? dpos[cls[s]] + s : 0u;
+        }
+
+        // 4. Every column through the staging buffer.
+        for (uint32_t j = 0; j < cols.ncols; ++j) {
+            if (j == cols.key) {
+                soa_write_keys<NW, R>(v, cols.out[j], slot, dest, valid,
+                                      reinterpret_cast<uint32_t*>(stage));
+                continue;
+            }
+            switch (cols.bytes[j]) {
+                case 1:
+                    soa_move_column<uint8_t, NW, R>(cols.in[j], cols.out[j], wbase, n, full, slot,
+                                                    dest, valid, stage, lane);
+                    break;
+                case 2:
+                    soa_move_column<uint16_t, NW, R>(cols.in[j], cols.out[j], wbase, n, full,
+                                                     slot, dest, valid, stage, lane);
+                    break;
+                case 4:
+                    soa_move_column<uint32_t, NW, R>(cols.in[j], cols.out[j], wbase, n, full,
+                                                     slot, dest, valid, stage, lane);
+                    break;
+                default:
+                    soa_move_column<uint64_t, NW, R>(cols.in[j], cols.out[j], wbase, n, full,
+                                                     slot, dest, valid, stage, lane);
+                    break;
+            }
+        }
+    }
+}
+
+}  // namespace mdiscr

@@ -1,0 +1,138 @@
+"""Struct-of-arrays partition on the GPU, bit-exact against the oracle
+(restated MdiscrHost_tuples.cu.h:81-112): the class comes from the key column,
+every column moves to the same index as raw bits, and stability is checked
+through an explicit original-index column."""
+import numpy as np
+import pytest
+
+import oracle
+from mdiscriminator_amd import workloads as W
+from tests.helpers import first_diff
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a):
+    import torch
+    a = np.ascontiguousarray(a)
+    signed = {np.dtype(np.uint8): np.uint8, np.dtype(np.uint16): np.int16,
+              np.dtype(np.uint32): np.int32, np.dtype(np.uint64): np.int64}
+    return torch.from_numpy(a.view(signed.get(a.dtype, a.dtype))).to("cuda:0")
+
+
+def host(t, like):
+    return t.cpu().numpy().view(like.dtype)
+
+
+def run_soa(ctx, cols, discr, key_col=0):
+    import torch
+    d_in = [dev(c) for c in cols]
+    d_out = [torch.empty_like(t) for t in d_in]
+    d_cnt = torch.full((discr.m,), -1, dtype=torch.int32, device="cuda:0")
+    ctx.partition_soa(d_in, d_out, discr, key_col=key_col, d_counts=d_cnt)
+    torch.cuda.synchronize()
+    return [host(o, c) for o, c in zip(d_out, cols)], d_cnt.cpu().numpy().view(np.uint32)
+
+
+def check_soa(ctx, cols, discr, key_col=0, label=""):
+    outs, cnt = run_soa(ctx, cols, discr, key_col)
+    ref_outs, ref_cnt = oracle.seq_mdiscr_soa(cols, discr.m, key_col=key_col, kind=discr.kind)
+    assert np.array_equal(cnt, ref_cnt), label
+    for j, (o, r) in enumerate(zip(outs, ref_outs)):
+        i = first_diff(o, r)
+        assert i < 0, f"{label} column {j} first mismatch at {i}"
+    return outs
+
+
+@pytest.mark.parametrize("m", [16, 32, 64])
+def test_soa_large_m_records(ctx, m):
+    """The large-m record layout: (int32 key, float32 payload bits, int32 index)."""
+    import mdiscriminator_amd as md
+    for n in (1, 4095, 4096, 4097, 100003, 1 << 20):
+        key = W.uniform_u31(n, 300 + m)
+        pay = W.normal_f32_bits(n, 400 + m)
+        idx = np.arange(n, dtype=np.uint32)
+        print(f"soa m={m} n={n}", flush=True)
+        outs = check_soa(ctx, [key, pay, idx], md.Mask(m), label=f"m={m} n={n}")
+        # stability through the index column
+        c = outs[0] & (m - 1)
+        for k in range(m):
+            s = outs[2][c == k]
+            assert np.all(np.diff(s.astype(np.int64)) > 0)
+
+
+def test_soa_nan_payload_bits_survive(ctx):
+    """Payloads move as integers: signalling/quiet NaN patterns and -0.0 keep their bits."""
+    import mdiscriminator_amd as md
+    n = 20000
+    key = W.uniform_u31(n, 1)
+    pay = np.array([0x7FA00001, 0xFFC00000, 0x80000000, 0x7F800000, 0x00000001],
+                   dtype=np.uint32)[np.arange(n) % 5]
+    check_soa(ctx, [key, pay], md.Mod(10), label="nan bits")
+
+
+def test_soa_reference_tuple_types(ctx):
+    """The reference's own tuple: (unsigned int, unsigned char, bool), class from column 1
+    (ModTuple<N>, MdiscrHost_tuples.cu.h:31-48), plus 2- and 8-byte columns and a key that
+    is not the first column."""
+    import mdiscriminator_amd as md
+    n = 77777
+    key = W.uniform_u31(n, 9)
+    u8 = (W.uniform_u31(n, 10) & 0xFF).astype(np.uint8)
+    b = (W.uniform_u31(n, 11) & 1).astype(np.uint8)
+    for m in (4, 5, 8):
+        check_soa(ctx, [key, u8, b], md.Mod(m), label=f"tuple m={m}")
+    u16 = (W.uniform_u31(n, 12) & 0xFFFF).astype(np.uint16)
+    u64 = W.splitmix64(n, 13)
+    check_soa(ctx, [u16, u64, key], md.Mod(13), key_col=2, label="key last")
+
+
+def test_soa_module_level_tuple_call(ctx):
+    import torch
+    import mdiscriminator_amd as md
+    n = 50000
+    key = W.uniform_u31(n, 21)
+    u8 = (key & 0xFF).astype(np.uint8)
+    b = (key >> 3 & 1).astype(np.uint8)
+    d1, d2, d3 = dev(key), dev(u8), dev(b)
+    o1, o2, o3 = torch.empty_like(d1), torch.empty_like(d2), torch.empty_like(d3)
+    counts = md.mdiscr_tuples(md.Mod(8), n, 1 << 16, d1, d2, d3, o1, o2, o3)
+    ref, ref_cnt = oracle.seq_mdiscr_soa([key, u8, b], 8)
+    assert counts == ref_cnt.tolist()
+    assert np.array_equal(host(o1, key), ref[0])
+    assert np.array_equal(host(o2, u8), ref[1])
+    assert np.array_equal(host(o3, b), ref[2])
+
+
+def test_soa_full_size_2p26(ctx):
+    """Config-size run: 2^26 records, m=64, checked against the oracle."""
+    import torch
+    import mdiscriminator_amd as md
+    n = 1 << 26
+    seed = 0x5EED0000 + 3
+    d_key = W.uniform_u31_torch(n, seed, "cuda:0")
+    key = d_key.cpu().numpy().view(np.uint32)
+    pay = W.normal_f32_bits(n, seed + 1)
+    d_pay = dev(pay)
+    d_idx = torch.arange(n, dtype=torch.int32, device="cuda:0")
+    outs = [torch.empty_like(d_key), torch.empty_like(d_pay), torch.empty_like(d_idx)]
+    d_cnt = torch.empty(64, dtype=torch.int32, device="cuda:0")
+    ctx.partition_soa([d_key, d_pay, d_idx], outs, md.Mask(64), d_counts=d_cnt)
+    torch.cuda.synchronize()
+    ref, ref_cnt = oracle.seq_mdiscr_soa([key, pay, np.arange(n, dtype=np.uint32)], 64,
+                                         kind=oracle.MASK)
+    assert np.array_equal(d_cnt.cpu().numpy().view(np.uint32), ref_cnt)
+    for o, r in zip(outs, ref):
+        assert first_diff(o.cpu().numpy().view(np.uint32), r) < 0
+
+
+def test_soa_invalid(ctx):
+    import torch
+    import mdiscriminator_amd as md
+    a = torch.zeros(100, dtype=torch.int32, device="cuda:0")
+    b8 = torch.zeros(100, dtype=torch.uint8, device="cuda:0")
+    o = torch.zeros(100, dtype=torch.int32, device="cuda:0")
+    with pytest.raises(md.MdiscrError):
+        ctx.partition_soa([b8, a], [torch.empty_like(b8), o], md.Mod4, key_col=0)  # 1-byte key
+    with pytest.raises(md.MdiscrError):
+        ctx.partition_soa([a], [a], md.Mod4)                                        # aliasing
