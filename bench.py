@@ -115,6 +115,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=1 << 26)
+    ap.add_argument("--profile-mode", action="store_true",
+                    help="for rocprofv3 counter runs: a few partition steps, then an "
+                         "elementwise copy of the same 2^28 uint32 (known 1 GiB read + "
+                         "1 GiB write) that calibrates FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -141,6 +145,17 @@ def main():
 
     def step_zipf():
         ctx.partition_u32(d_zipf, d_out, md.Mod8, d_counts=d_cnt, stream=stream)
+
+    if args.profile_mode:
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        for _ in range(args.steps):
+            torch.bitwise_xor(d_in, 0, out=d_out)
+        torch.cuda.synchronize()
+        log("profile mode done")
+        ctx.close()
+        return
 
     log(f"timing {args.steps} steps after {args.warmup} warmup")
     dt = time_steps(ctx, step, args.steps, args.warmup)

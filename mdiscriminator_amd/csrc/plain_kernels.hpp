@@ -170,8 +170,10 @@ __global__ __launch_bounds__(NW* kWave) void k_chunk_count(
 template <int THREADS, int PER>
 __global__ __launch_bounds__(THREADS) void k_tile_scan(
     uint32_t* __restrict__ counts, uint32_t T, uint32_t ld, uint32_t* __restrict__ totals,
-    uint32_t* __restrict__ user_counts) {
+    uint32_t* __restrict__ user_counts, const uint32_t* __restrict__ rows_dev = nullptr) {
     static_assert(PER % 4 == 0, "vector loads");
+    // Row count from device memory (a planned queue): rows plus the zero after them.
+    if (rows_dev) T = *rows_dev + 1u;
     constexpr int NWAVE = THREADS / kWave;
     __shared__ uint32_t wsum[NWAVE];
     uint32_t* row = counts + (size_t)blockIdx.x * ld;
@@ -225,7 +227,7 @@ __global__ __launch_bounds__(THREADS) void k_tile_scan(
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        totals[blockIdx.x] = carry;
+        if (totals) totals[blockIdx.x] = carry;
         if (user_counts) user_counts[blockIdx.x] = carry;
     }
 }

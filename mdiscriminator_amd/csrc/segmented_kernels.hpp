@@ -1,0 +1,348 @@
+// segmented_kernels.hpp -- batched segmented m-way stable partition on gfx950.
+//
+// Replaces the reference's 11-launch segmented pipeline (mdiscrSegm,
+// MdiscrSegmHost.cu.h:36-230; kernels MdiscrSegmKernels.cu.h:9-114), whose
+// semantics are "plain m-discrimination of every segment in place of its
+// range" (validated per segment, MdiscrSegmTest.cu:21-38).
+//
+// Work is described by a device table of segment descriptors (input and
+// output pointers, length, discriminator), so one launch can carry segments of
+// one array (offsets) or of unrelated buffers (a serving batch).  A planning
+// kernel builds the work queue on the device:
+//   W item : up to NW consecutive short segments (len <= 64*R), one per wave;
+//            each wave partitions its segment alone (no block barrier).
+//   T item : tile j of a segment with len > 64*R.  A one-tile segment is
+//            written straight from LDS; tiles of longer segments take their
+//            offsets from a count pass + scan over "rows" (one row per tile
+//            of a multi-tile segment, consecutive within a segment).
+// count and partition are persistent: a fixed grid walks the queue, whose
+// length is read from device memory, so the whole sequence can be captured
+// in a graph once and replayed with new contents.
+#pragma once
+
+#include "plain_kernels.hpp"
+
+namespace mdiscr {
+
+struct SegDesc {
+    const uint32_t* in;
+    uint32_t* out;
+    uint32_t* counts;   // m class counts of this segment, or null
+    uint32_t len;
+    uint32_t m;
+    mdiscr_divmagic g;  // x % m
+};
+
+struct SegItem {
+    uint32_t seg;     // (first) segment
+    uint32_t j;       // T: tile index; W: number of segments
+    uint32_t ntiles;  // T: tiles of the segment (>= 1); W: 0
+    uint32_t row;     // T with ntiles > 1: count row of this tile
+};
+
+struct SegPlan {
+    uint32_t n_items;
+    uint32_t n_rows;
+};
+
+template <bool MASK>
+struct SegDiscr;
+template <>
+struct SegDiscr<true> {
+    static __device__ __forceinline__ DiscrMask make(const SegDesc& d) { return DiscrMask{d.m - 1u}; }
+};
+template <>
+struct SegDiscr<false> {
+    static __device__ __forceinline__ DiscrMod make(const SegDesc& d) { return DiscrMod{d.g}; }
+};
+
+// ---------------------------------------------------------------------------
+// One descriptor per segment of a single array given by offsets.
+__global__ void k_seg_desc_from_offsets(const uint64_t* __restrict__ off, uint32_t nseg,
+                                        const uint32_t* in, uint32_t* out,
+                                        uint32_t* seg_counts, uint32_t m, mdiscr_divmagic g,
+                                        SegDesc* __restrict__ desc) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nseg) return;
+    const uint64_t a = off[s], b = off[s + 1];
+    SegDesc d;
+    d.in = in + a;
+    d.out = out + a;
+    d.counts = seg_counts ? seg_counts + (size_t)s * m : nullptr;
+    d.len = (uint32_t)(b - a);
+    d.m = m;
+    d.g = g;
+    desc[s] = d;
+}
+
+// ---------------------------------------------------------------------------
+// Planning: one workgroup of THREADS walks the segments in passes, with
+// block scans for the item and row counts and a max-scan for the last long
+// segment (which bounds each run of short segments).
+template <int THREADS>
+__device__ __forceinline__ uint32_t block_excl_sum(uint32_t v, uint32_t* wsum, uint32_t& total) {
+    constexpr int NWAVE = THREADS / kWave;
+    const uint32_t lane = lane_id(), wave = threadIdx.x / kWave;
+    const uint32_t wex = wave_excl_scan(v, lane);
+    if (lane == kWave - 1) wsum[wave] = wex + v;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < NWAVE; ++w) {
+        pre += (w < (int)wave) ? wsum[w] : 0u;
+        tot += wsum[w];
+    }
+    __syncthreads();
+    total = tot;
+    return pre + wex;
+}
+
+template <int THREADS>
+__device__ __forceinline__ int block_incl_max(int v, int* wmax) {
+    constexpr int NWAVE = THREADS / kWave;
+    const uint32_t lane = lane_id(), wave = threadIdx.x / kWave;
+    int x = v;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const int y = __shfl_up(x, d, kWave);
+        if (lane >= (uint32_t)d) x = max(x, y);
+    }
+    if (lane == kWave - 1) wmax[wave] = x;
+    __syncthreads();
+    int pre = -1;
+#pragma unroll
+    for (int w = 0; w < NWAVE; ++w)
+        if (w < (int)wave) pre = max(pre, wmax[w]);
+    __syncthreads();
+    return max(pre, x);
+}
+
+template <int THREADS, int NW, int R>
+__global__ __launch_bounds__(THREADS) void k_seg_plan(const SegDesc* __restrict__ desc,
+                                                      uint32_t nseg, SegItem* __restrict__ items,
+                                                      SegPlan* __restrict__ plan,
+                                                      uint32_t* __restrict__ counts, uint32_t ld,
+                                                      uint32_t mrows) {
+    constexpr uint32_t WT = kWave * R;          // one wave's elements
+    constexpr uint32_t TILE = NW * kWave * R;   // one workgroup's elements
+    __shared__ uint32_t wsum[THREADS / kWave];
+    __shared__ int wmax[THREADS / kWave];
+    uint32_t item_carry = 0, row_carry = 0;
+    int last_long = -1;  // index of the last long segment seen so far
+    for (uint32_t base = 0; base < nseg; base += THREADS) {
+        const uint32_t s = base + threadIdx.x;
+        const bool in_range = s < nseg;
+        const uint32_t len = in_range ? desc[s].len : 0u;
+        const bool is_short = in_range && len <= WT;
+        const uint32_t tiles = (in_range && !is_short) ? (len + TILE - 1) / TILE : 0u;
+        const int lng = block_incl_max<THREADS>((in_range && !is_short) ? (int)s : -1, wmax);
+        const int run_prev = max(lng, last_long);     // last long segment at or before s
+        const bool leader = is_short && ((int)s - run_prev - 1) % NW == 0;
+        uint32_t gsz = 0;
+        if (leader) {
+            gsz = 1;
+            while (gsz < (uint32_t)NW && s + gsz < nseg && desc[s + gsz].len <= WT) ++gsz;
+        }
+        const uint32_t nitems = tiles + (leader ? 1u : 0u);
+        const uint32_t nrows = tiles > 1 ? tiles : 0u;
+        uint32_t itot, rtot;
+        const uint32_t ibase = item_carry + block_excl_sum<THREADS>(nitems, wsum, itot);
+        const uint32_t rbase = row_carry + block_excl_sum<THREADS>(nrows, wsum, rtot);
+        if (leader) items[ibase] = SegItem{s, gsz, 0u, 0u};
+        for (uint32_t j = 0; j < tiles; ++j)
+            items[ibase + j] = SegItem{s, j, tiles, tiles > 1 ? rbase + j : 0u};
+        item_carry += itot;
+        row_carry += rtot;
+        // the last long segment of this pass, for the next pass
+        if (threadIdx.x == THREADS - 1) wmax[0] = max(lng, last_long);
+        __syncthreads();
+        last_long = wmax[0];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *plan = SegPlan{item_carry, row_carry};
+    // A zero after the last row: the scan's exclusive prefix there is the total.
+    for (uint32_t c = threadIdx.x; c < mrows; c += THREADS) counts[(size_t)c * ld + row_carry] = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// count: class counts of every tile of a multi-tile segment, counts[c*ld+row].
+template <int NW, int R, int NBITS, bool MASK>
+__global__ __launch_bounds__(NW* kWave) void k_seg_count(const SegDesc* __restrict__ desc,
+                                                         const SegItem* __restrict__ items,
+                                                         const SegPlan* __restrict__ plan,
+                                                         uint32_t* __restrict__ counts,
+                                                         uint32_t ld) {
+    constexpr int MC = 1 << NBITS;
+    constexpr int THREADS = NW * kWave;
+    constexpr uint32_t TILE = NW * kWave * R;
+    constexpr bool PACKED = LaneCounts<NBITS>::kPacked;
+    __shared__ uint32_t hist[NW][MC];
+    const uint32_t lane = lane_id();
+    const uint32_t wave = threadIdx.x / kWave;
+    const uint32_t n_items = plan->n_items;
+    for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x) {
+        const SegItem it = items[i];
+        if (it.ntiles <= 1) continue;  // block-uniform
+        const SegDesc d = desc[it.seg];
+        const auto dis = SegDiscr<MASK>::make(d);
+        __syncthreads();  // hist reuse
+        if (!PACKED) {
+            for (int c = lane; c < MC; c += kWave) hist[wave][c] = 0;
+            __builtin_amdgcn_wave_barrier();
+        }
+        const uint32_t tstart = it.j * TILE;
+        const bool full = tstart + TILE <= d.len;
+        uint32_t v[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const uint32_t idx = tstart + k * THREADS + threadIdx.x;
+            v[k] = (full || idx < d.len) ? __builtin_nontemporal_load(d.in + idx) : 0u;
+        }
+        LaneCounts<NBITS> lc;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const uint32_t idx = tstart + k * THREADS + threadIdx.x;
+            if (full || idx < d.len) {
+                const uint32_t c = dis(v[k]);
+                if (PACKED) lc.add(c);
+                else atomicAdd(&hist[wave][c], 1u);
+            }
+        }
+        if (PACKED) {
+            const uint32_t cnt = lc.wave_class_count(lane);
+            if (lane < (uint32_t)MC) hist[wave][lane] = cnt;
+        }
+        __syncthreads();
+        for (uint32_t c = threadIdx.x; c < d.m; c += THREADS) {
+            uint32_t s = 0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) s += hist[w][c];
+            counts[(size_t)c * ld + it.row] = s;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// partition: walks the queue.  STAGED: place each tile in LDS grouped by class
+// and write contiguous runs; !STAGED: store every element straight from
+// registers to its destination (no staging pass; for few classes the
+// destinations of one wave store form a handful of contiguous runs).
+template <int NW, int R, int NBITS, bool MASK, bool STAGED>
+__global__ __launch_bounds__(NW* kWave) void k_seg_partition(
+    const SegDesc* __restrict__ desc, const SegItem* __restrict__ items,
+    const SegPlan* __restrict__ plan, const uint32_t* __restrict__ prefix, uint32_t ld) {
+    constexpr int MC = 1 << NBITS;
+    constexpr int THREADS = NW * kWave;
+    constexpr uint32_t WT = kWave * R;
+    constexpr uint32_t TILE = NW * WT;
+    __shared__ uint32_t stage[TILE];
+    __shared__ uint32_t cursor[NW][MC];
+    __shared__ uint32_t dpos[MC];
+
+    const uint32_t lane = lane_id();
+    const uint32_t wave = threadIdx.x / kWave;
+    const uint32_t n_items = plan->n_items;
+    for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x) {
+        const SegItem it = items[i];
+        __syncthreads();  // LDS reuse across items
+        if (it.ntiles == 0) {
+            // ---- W item: wave w partitions segment it.seg + w on its own.
+            if (wave < it.j) {
+                const SegDesc d = desc[it.seg + wave];
+                const auto dis = SegDiscr<MASK>::make(d);
+                uint32_t v[R], packed[R];
+                load_rounds<R, false>(d.in, v, 0u, d.len, lane);
+                for (int c = lane; c < MC; c += kWave) cursor[wave][c] = 0;
+                __builtin_amdgcn_wave_barrier();
+                rank_rounds<R, NBITS, false>(v, packed, cursor[wave], 0u, d.len, d.m, lane, dis);
+                // class bases inside the segment (lane = class)
+                const uint32_t cnt = lane < (uint32_t)MC ? cursor[wave][lane] : 0u;
+                const uint32_t toff = wave_excl_scan(cnt, lane);
+                if (lane < (uint32_t)MC) cursor[wave][lane] = toff;
+                if (d.counts && lane < d.m)
+                    d.counts[lane] = cnt - (lane == d.m - 1u ? WT - d.len : 0u);
+                __builtin_amdgcn_wave_barrier();
+                uint32_t* st = stage + wave * WT;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const uint32_t c = packed[r] >> 16;
+                    const uint32_t slot = cursor[wave][c] + (packed[r] & 0xFFFFu);
+                    if (STAGED) st[slot] = v[r];
+                    else if (slot < d.len) __builtin_nontemporal_store(v[r], d.out + slot);
+                }
+                if (STAGED) {
+                    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                    for (int k = 0; k < R; ++k) {
+                        const uint32_t s = k * kWave + lane;
+                        if (s < d.len) __builtin_nontemporal_store(st[s], d.out + s);
+                    }
+                }
+            }
+            continue;
+        }
+        // ---- T item: tile it.j of segment it.seg.
+        const SegDesc d = desc[it.seg];
+        const auto dis = SegDiscr<MASK>::make(d);
+        const uint32_t tstart = it.j * TILE;
+        const uint32_t wbase = tstart + wave * WT;
+        const bool full = tstart + TILE <= d.len;
+        const uint32_t valid = full ? TILE : d.len - tstart;
+        uint32_t pj = 0, p0 = 0, pe = 0;
+        if (wave == 0 && it.ntiles > 1 && lane < d.m) {
+            const uint32_t row0 = it.row - it.j;
+            const uint32_t* pr = prefix + (size_t)lane * ld;
+            pj = pr[it.row];
+            p0 = pr[row0];
+            pe = pr[row0 + it.ntiles];
+        }
+        uint32_t v[R], packed[R];
+        if (full) load_rounds<R, true>(d.in, v, wbase, d.len, lane);
+        else load_rounds<R, false>(d.in, v, wbase, d.len, lane);
+        for (int c = lane; c < MC; c += kWave) cursor[wave][c] = 0;
+        __builtin_amdgcn_wave_barrier();
+        if (full) rank_rounds<R, NBITS, true>(v, packed, cursor[wave], wbase, d.len, d.m, lane, dis);
+        else rank_rounds<R, NBITS, false>(v, packed, cursor[wave], wbase, d.len, d.m, lane, dis);
+        __syncthreads();
+        if (wave == 0) {
+            uint32_t running;
+            if (it.ntiles > 1) {
+                // segment-relative: class base from the segment totals, plus
+                // this tile's prefix within the segment
+                running = wave_excl_scan(pe - p0, lane) + (pj - p0);
+                if (it.j == 0 && d.counts && lane < d.m) d.counts[lane] = pe - p0;
+            } else {
+                running = 0;  // one tile: LDS order is the output order
+            }
+            const uint32_t before = running;
+            tile_offsets<NW, MC>(cursor, dpos, running, lane);
+            if (it.ntiles == 1) {
+                if (lane < (uint32_t)MC) dpos[lane] = 0u;
+                const uint32_t tot = running - before;
+                if (d.counts && lane < d.m)
+                    d.counts[lane] = tot - (lane == d.m - 1u ? TILE - valid : 0u);
+            }
+        }
+        __syncthreads();
+        if (STAGED) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const uint32_t c = packed[r] >> 16;
+                stage[cursor[wave][c] + (packed[r] & 0xFFFFu)] = v[r];
+            }
+            __syncthreads();
+            if (full) write_runs<R, THREADS, (R < 8 ? R : 8), true>(stage, dpos, d.out, valid, dis);
+            else write_runs<R, THREADS, (R < 8 ? R : 8), false>(stage, dpos, d.out, valid, dis);
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const uint32_t c = packed[r] >> 16;
+                const uint32_t slot = cursor[wave][c] + (packed[r] & 0xFFFFu);
+                if (full || slot < valid)
+                    __builtin_nontemporal_store(v[r], d.out + (dpos[c] + slot));
+            }
+        }
+    }
+}
+
+}  // namespace mdiscr

@@ -1,0 +1,201 @@
+"""Batched segmented partition on the GPU, bit-exact against the oracle applied
+per segment (MdiscrSegmTest.cu:21-38), through both entry points: segment
+offsets (mdiscr_partition_segmented) and the reference's flag arrays
+(mdiscr_segm_flags, i.e. mdiscrSegm(n, in, segment_sizes, out, new_sizes))."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from mdiscriminator_amd import workloads as W
+from tests.helpers import first_diff, from_dev, to_dev
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+KAT = json.load(open(os.path.join(HERE, "golden", "kat.json")))
+NUM_CONFIGS = 8
+
+
+def run_offsets(ctx, x, off, discr, stream=None, counts=True):
+    import torch
+    d_in = to_dev(x)
+    d_out = torch.full_like(d_in, -1)
+    d_off = torch.from_numpy(off.astype(np.int64)).to("cuda:0")
+    nseg = len(off) - 1
+    d_cnt = torch.full((max(nseg * discr.m, 1),), -1, dtype=torch.int32, device="cuda:0") \
+        if counts else None
+    ctx.partition_segmented(d_in, d_out, d_off, discr, h_seg_off=off.astype(np.uint64),
+                            d_seg_counts=d_cnt, stream=stream)
+    torch.cuda.synchronize()
+    cnt = from_dev(d_cnt)[:nseg * discr.m].reshape(nseg, discr.m) if counts else None
+    return from_dev(d_out), cnt
+
+
+def check_offsets(ctx, x, off, discr, label="", stream=None):
+    out, cnt = run_offsets(ctx, x, off, discr, stream=stream)
+    ref, ref_cnt = oracle.seq_mdiscr_segm(x, off, discr.m, discr.kind)
+    i = first_diff(out, ref)
+    assert i < 0, f"{label}: first mismatch at {i} ({out[i]} vs {ref[i]})"
+    assert np.array_equal(cnt, ref_cnt), f"{label}: segment counts differ"
+
+
+def lengths_mix(seed, nseg, median, sigma=1.5, zeros=True):
+    L = W.lognormal_lengths(nseg, seed, median=median, sigma=sigma, lo=0 if zeros else 1)
+    if zeros:
+        L[::17] = 0
+    return L
+
+
+def test_reference_kat_flags(ctx):
+    """haskell_code/testsSegm.hs inputs; expected from the reference SeqMdiscr per segment."""
+    import torch
+    import mdiscriminator_amd as md
+    case = KAT["segmented"][0]
+    x = np.array(case["input"], dtype=np.uint32)
+    f = np.array(case["segment_sizes"], dtype=np.int32)
+    d_in = to_dev(x)
+    d_out = torch.empty_like(d_in)
+    d_new = torch.full_like(d_in, -7)
+    md.mdiscr_segm(md.Mod(case["m"]), len(x), d_in, torch.from_numpy(f).to("cuda:0"), d_out,
+                   d_new)
+    assert from_dev(d_out).tolist() == case["out"]
+    assert d_new.cpu().tolist() == case["new_sizes"]
+
+
+@pytest.mark.parametrize("m", [1, 2, 3, 4, 7, 8, 16, 33, 64])
+def test_segmented_lognormal_mix(ctx, m):
+    import mdiscriminator_amd as md
+    for median, nseg in ((30, 400), (900, 300), (5000, 120), (40000, 20)):
+        L = lengths_mix(m * 100 + median, nseg, median)
+        off = W.offsets_from_lengths(L)
+        x = W.uniform_u31(int(off[-1]), median + m)
+        print(f"segmented m={m} median={median} n={off[-1]}", flush=True)
+        check_offsets(ctx, x, off, md.Mod(m), f"m={m} median={median}")
+
+
+@pytest.mark.parametrize("cfg", list(range(NUM_CONFIGS)))
+def test_every_tuned_config(ctx, cfg):
+    """Every candidate configuration of the autotuner gives the exact result."""
+    import mdiscriminator_amd as md
+    os.environ["MDISCR_SEG_CONFIG"] = str(cfg)
+    try:
+        for m in (2, 4, 8, 16):
+            L = lengths_mix(cfg * 31 + m, 300, 2000)
+            L[5] = 70000    # a many-tile segment
+            L[6] = 1        # and neighbours of every size
+            L[7] = 2047
+            off = W.offsets_from_lengths(L)
+            x = W.uniform_u31(int(off[-1]), cfg + m)
+            check_offsets(ctx, x, off, md.Mask(m), f"cfg={cfg} m={m}")
+    finally:
+        del os.environ["MDISCR_SEG_CONFIG"]
+
+
+def test_segment_shapes(ctx):
+    import mdiscriminator_amd as md
+    shapes = {
+        "one segment": np.array([123457]),
+        "all length one": np.ones(5000, dtype=np.int64),
+        "all empty but one": np.array([0, 0, 0, 9, 0, 0]),
+        "exact tiles": np.array([4096, 8192, 1024, 64, 4096 * 5]),
+        "tile +- 1": np.array([4095, 4097, 1023, 1025, 63, 65, 8191, 8193]),
+    }
+    for name, L in shapes.items():
+        off = W.offsets_from_lengths(L)
+        x = W.uniform_u31(int(off[-1]), len(L))
+        for m in (4, 5):
+            check_offsets(ctx, x, off, md.Mod(m), f"{name} m={m}")
+
+
+def test_graph_replay_sees_new_contents(ctx):
+    """A call on a non-null stream is captured once and replayed; a replay
+    with new input contents in the same buffers gives the new result."""
+    import torch
+    import mdiscriminator_amd as md
+    L = lengths_mix(5, 500, 3000)
+    off = W.offsets_from_lengths(L)
+    n = int(off[-1])
+    s = torch.cuda.Stream()
+    d_in = torch.empty(n, dtype=torch.int32, device="cuda:0")
+    d_out = torch.empty_like(d_in)
+    d_off = torch.from_numpy(off.astype(np.int64)).to("cuda:0")
+    for rep in range(4):
+        x = W.uniform_u31(n, 1000 + rep)
+        d_in.copy_(torch.from_numpy(x.view(np.int32)))
+        torch.cuda.synchronize()
+        with torch.cuda.stream(s):
+            ctx.partition_segmented(d_in, d_out, d_off, md.Mod4, stream=s)
+        s.synchronize()
+        torch.cuda.synchronize()
+        ref, _ = oracle.seq_mdiscr_segm(x, off, 4)
+        assert first_diff(from_dev(d_out), ref) < 0, f"replay {rep}"
+
+
+def test_flags_interface_random(ctx):
+    import torch
+    import mdiscriminator_amd as md
+    L = lengths_mix(9, 3000, 700, zeros=False)
+    n = int(L.sum())
+    x = W.uniform_u31(n, 10)
+    f = W.flags_from_lengths(L, n)
+    d_in = to_dev(x)
+    d_out = torch.empty_like(d_in)
+    d_new = torch.empty_like(d_in)
+    ctx.segm_flags(md.Mod(6), n, d_in, torch.from_numpy(f).to("cuda:0"), d_out, d_new)
+    torch.cuda.synchronize()
+    ref, ref_new = oracle.segm_flags(x, f, 6)
+    assert first_diff(from_dev(d_out), ref) < 0
+    assert np.array_equal(d_new.cpu().numpy(), ref_new)
+
+
+def test_flags_interface_rejects_bad_flags(ctx):
+    import torch
+    import mdiscriminator_amd as md
+    x = to_dev(np.arange(10, dtype=np.uint32))
+    o = torch.empty_like(x)
+    ns = torch.empty_like(x)
+    for bad in ([3, 0, 0, 4, 0, 0, 0, 5, 0, 0],    # last segment runs past n
+                [0, 3, 0, 0, 6, 0, 0, 0, 0, 0],    # no segment starts at 0
+                [2, 0, 3, 0, 0, 0, 4, 0, 0, 0],    # length disagrees with next start
+                [10, 0, 0, 0, 0, -1, 0, 0, 0, 0]):  # negative flag
+        with pytest.raises(md.MdiscrError):
+            ctx.segm_flags(md.Mod4, 10, x, torch.tensor(bad, dtype=torch.int32,
+                                                         device="cuda:0"), o, ns)
+
+
+def test_segmented_invalid_offsets(ctx):
+    import torch
+    import mdiscriminator_amd as md
+    d_in = to_dev(np.arange(100, dtype=np.uint32))
+    d_out = torch.empty_like(d_in)
+    bad = np.array([0, 50, 40, 100], dtype=np.uint64)
+    with pytest.raises(md.MdiscrError):
+        ctx.partition_segmented(d_in, d_out, torch.from_numpy(bad.astype(np.int64)).to("cuda:0"),
+                                md.Mod4, h_seg_off=bad)
+
+
+def test_full_size_10k_lognormal(ctx):
+    """The batched workload at its stated parameters: 10,000 segments with
+    lognormal lengths (median 4096, sigma 1.5, clipped to [1, 4Mi]), m=4."""
+    import torch
+    import mdiscriminator_amd as md
+    L = W.lognormal_lengths(10000, W.SEED_BASE + 4)
+    off = W.offsets_from_lengths(L)
+    n = int(off[-1])
+    print(f"10k lognormal: n={n}", flush=True)
+    d_in = W.uniform_u31_torch(n, W.SEED_BASE + 4, "cuda:0")
+    d_out = torch.empty_like(d_in)
+    d_off = torch.from_numpy(off.astype(np.int64)).to("cuda:0")
+    d_cnt = torch.empty(10000 * 4, dtype=torch.int32, device="cuda:0")
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        ctx.partition_segmented(d_in, d_out, d_off, md.Mod4, d_seg_counts=d_cnt, stream=s)
+        ctx.partition_segmented(d_in, d_out, d_off, md.Mod4, d_seg_counts=d_cnt, stream=s)
+    s.synchronize()
+    x = from_dev(d_in)
+    ref, ref_cnt = oracle.seq_mdiscr_segm(x, off, 4)
+    assert first_diff(from_dev(d_out), ref) < 0
+    assert np.array_equal(from_dev(d_cnt).reshape(10000, 4), ref_cnt)
