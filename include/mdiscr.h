@@ -135,6 +135,70 @@ mdiscr_status mdiscr_segm_flags(mdiscr_ctx* ctx, uint64_t n,
                                 uint32_t* d_out, int32_t* d_new_sizes,
                                 mdiscr_discr d, void* stream);
 
+/* ---- batched independent requests ----------------------------------------- */
+
+/* One partition request: its own buffers, length and discriminator. */
+typedef struct {
+    const uint32_t* d_in;
+    uint32_t* d_out;
+    uint64_t n;
+    mdiscr_discr d;
+    uint32_t* d_counts; /* m entries, may be NULL */
+} mdiscr_request;
+
+/* Runs nreq independent requests as ONE segmented launch (each request is a
+ * segment with its own input, output and m).  No reference counterpart: it
+ * is the batched form that mdiscrSegm (MdiscrSegmHost.cu.h:36-42) provides
+ * for segments of one array, generalised to separate buffers.  reqs is a
+ * HOST array; the call is stream-ordered. */
+mdiscr_status mdiscr_partition_batch(mdiscr_ctx* ctx, const mdiscr_request* reqs,
+                                     uint32_t nreq, void* stream);
+
+/* ---- serving ----------------------------------------------------------------
+ * No reference counterpart (the reference has no batching or serving).  A
+ * server owns nstreams contexts and streams and one scheduler thread.
+ * Requests arrive with their input already in device memory; each is
+ * admitted once its output buffer fits the device-memory budget (FIFO, so a
+ * large request is not overtaken), and every admitted request that is not yet
+ * on the GPU joins the next batch: a single mdiscr_partition_batch launch on
+ * a free stream, while earlier batches still run on the others.  Outputs stay
+ * resident until released.  Thread-safe: any host thread may submit, wait and
+ * release. */
+typedef struct mdiscr_server mdiscr_server;
+
+typedef struct {
+    int device;
+    uint64_t budget_bytes;       /* device memory for request outputs */
+    uint32_t nstreams;           /* batches in flight at once (1..16) */
+    uint32_t max_batch_requests; /* requests per launch */
+    uint64_t max_batch_elems;    /* elements per launch */
+    uint32_t max_pending;        /* submit returns MDISCR_EBUSY beyond this */
+} mdiscr_server_config;
+
+typedef struct {
+    uint64_t submitted, completed, batches, elems;
+    double p50_s, p99_s, mean_s, max_s; /* submit -> results ready */
+    uint64_t peak_bytes;                /* most output memory held at once */
+    double mean_batch_requests;
+} mdiscr_server_stats;
+
+mdiscr_status mdiscr_server_create(mdiscr_server** out, const mdiscr_server_config* cfg);
+mdiscr_status mdiscr_server_destroy(mdiscr_server* srv);
+/* Queues a request; *ticket identifies it.  d_in must stay valid until the
+ * request completes. */
+mdiscr_status mdiscr_server_submit(mdiscr_server* srv, const uint32_t* d_in, uint64_t n,
+                                   mdiscr_discr d, uint64_t* ticket);
+/* *done = 1 once the request's output and counts are ready. */
+mdiscr_status mdiscr_server_poll(mdiscr_server* srv, uint64_t ticket, int* done);
+/* Blocks until the request completes; returns its device output (valid until
+ * release), its class counts (h_counts, m entries, may be NULL) and its
+ * latency from submit to completion. */
+mdiscr_status mdiscr_server_wait(mdiscr_server* srv, uint64_t ticket, uint32_t** d_out,
+                                 uint32_t* h_counts, double* latency_s);
+/* Returns the request's output buffer to the budget. */
+mdiscr_status mdiscr_server_release(mdiscr_server* srv, uint64_t ticket);
+mdiscr_status mdiscr_server_stats_get(mdiscr_server* srv, mdiscr_server_stats* out);
+
 /* ---- output format (HelpersCommon.h:295-311) --------------------------- */
 
 /* d_sizes[0,n) = 0 except d_sizes[off[k]] = counts[k] for non-empty k, from

@@ -27,7 +27,7 @@ from . import _lib
 from ._lib import MdiscrError, DISCR_MOD, DISCR_MASK, MAX_CLASSES, MAX_ELEMS
 
 __all__ = [
-    "Discr", "Mod", "Mask", "Mod4", "Mod8", "Context", "MdiscrError", "default_context",
+    "Discr", "Mod", "Mask", "Mod4", "Mod8", "Context", "Server", "MdiscrError", "default_context",
     "mdiscr", "mdiscr_tuples", "mdiscr_segm", "sizes_from_counts", "MAX_CLASSES",
 ]
 
@@ -155,6 +155,16 @@ class Context:
                                         _stream(stream))
         self._check(st, "mdiscr_segm_flags")
 
+    def partition_batch(self, requests, stream=None):
+        """requests: list of (d_in, d_out, discr[, d_counts]) tensors; one launch."""
+        arr = (_lib.MdiscrRequest * len(requests))()
+        for i, r in enumerate(requests):
+            d_in, d_out, discr = r[0], r[1], r[2]
+            arr[i] = _lib.MdiscrRequest(_ptr(d_in), _ptr(d_out), d_in.numel(), discr.c(),
+                                        _ptr(r[3]) if len(r) > 3 else None)
+        st = self.lib.mdiscr_partition_batch(self.h, arr, len(requests), _stream(stream))
+        self._check(st, "mdiscr_partition_batch")
+
     # -- output format ---------------------------------------------------------
     def sizes_from_counts(self, d_counts, m, n, d_sizes, stream=None):
         st = self.lib.mdiscr_sizes_from_counts(self.h, _ptr(d_counts), m, n, _ptr(d_sizes),
@@ -174,6 +184,89 @@ class Context:
 
     def sync(self, stream=None):
         self._check(self.lib.mdiscr_sync(self.h, _stream(stream)), "mdiscr_sync")
+
+
+class Server:
+    """Continuous-batching server (mdiscr_server_*): submit device-resident
+    inputs, wait for device outputs; outputs count against the budget until
+    released."""
+
+    def __init__(self, budget_bytes, nstreams=4, max_batch_requests=1024,
+                 max_batch_elems=1 << 28, max_pending=0, device=0):
+        self.lib = _lib.load()
+        cfg = _lib.ServerConfig(device, budget_bytes, nstreams, max_batch_requests,
+                                max_batch_elems, max_pending)
+        h = ctypes.c_void_p()
+        st = self.lib.mdiscr_server_create(ctypes.byref(h), ctypes.byref(cfg))
+        if st != _lib.OK:
+            raise MdiscrError(st, "mdiscr_server_create")
+        self.h = h
+        self._m = {}
+
+    def close(self):
+        if self.h:
+            self.lib.mdiscr_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, d_in, discr, n=None):
+        t = ctypes.c_uint64()
+        n = d_in.numel() if n is None else n
+        st = self.lib.mdiscr_server_submit(self.h, _ptr(d_in), n, discr.c(), ctypes.byref(t))
+        if st != _lib.OK:
+            raise MdiscrError(st, "mdiscr_server_submit")
+        self._m[t.value] = (discr.m, n)
+        return t.value
+
+    def poll(self, ticket):
+        done = ctypes.c_int()
+        st = self.lib.mdiscr_server_poll(self.h, ticket, ctypes.byref(done))
+        if st != _lib.OK:
+            raise MdiscrError(st, "mdiscr_server_poll")
+        return bool(done.value)
+
+    def wait(self, ticket):
+        """-> (device output pointer, class counts, latency seconds)."""
+        m, n = self._m[ticket]
+        p = ctypes.c_void_p()
+        cnt = (ctypes.c_uint32 * m)()
+        lat = ctypes.c_double()
+        st = self.lib.mdiscr_server_wait(self.h, ticket, ctypes.byref(p), cnt, ctypes.byref(lat))
+        if st != _lib.OK:
+            raise MdiscrError(st, "mdiscr_server_wait")
+        return p.value, list(cnt), lat.value
+
+    def output_tensor(self, ticket, ptr):
+        """Copies the request's device output into a new int32 tensor."""
+        import torch
+        m, n = self._m[ticket]
+        out = torch.empty(n, dtype=torch.int32, device="cuda:0")
+        if n:
+            from ctypes import CDLL
+            hip = CDLL("libamdhip64.so")
+            hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+            rc = hip.hipMemcpy(ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(ptr), n * 4, 3)
+            if rc != 0:
+                raise MdiscrError(_lib.EDEVICE, f"hipMemcpy {rc}")
+        return out
+
+    def release(self, ticket):
+        st = self.lib.mdiscr_server_release(self.h, ticket)
+        if st != _lib.OK:
+            raise MdiscrError(st, "mdiscr_server_release")
+        self._m.pop(ticket, None)
+
+    def stats(self):
+        s = _lib.ServerStats()
+        st = self.lib.mdiscr_server_stats_get(self.h, ctypes.byref(s))
+        if st != _lib.OK:
+            raise MdiscrError(st, "mdiscr_server_stats_get")
+        return {f: getattr(s, f) for f, _ in _lib.ServerStats._fields_}
 
 
 _default = {}

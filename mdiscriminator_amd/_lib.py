@@ -19,6 +19,25 @@ class MdiscrDiscr(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_uint32), ("m", ctypes.c_uint32)]
 
 
+class MdiscrRequest(ctypes.Structure):
+    _fields_ = [("d_in", ctypes.c_void_p), ("d_out", ctypes.c_void_p), ("n", ctypes.c_uint64),
+                ("d", MdiscrDiscr), ("d_counts", ctypes.c_void_p)]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("budget_bytes", ctypes.c_uint64),
+                ("nstreams", ctypes.c_uint32), ("max_batch_requests", ctypes.c_uint32),
+                ("max_batch_elems", ctypes.c_uint64), ("max_pending", ctypes.c_uint32)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64),
+                ("batches", ctypes.c_uint64), ("elems", ctypes.c_uint64),
+                ("p50_s", ctypes.c_double), ("p99_s", ctypes.c_double),
+                ("mean_s", ctypes.c_double), ("max_s", ctypes.c_double),
+                ("peak_bytes", ctypes.c_uint64), ("mean_batch_requests", ctypes.c_double)]
+
+
 class MdiscrError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__(f"mdiscr status {status}: {msg}")
@@ -44,6 +63,15 @@ SIGNATURES = {
                                           _vp]),
     "mdiscr_segm_flags": (_int, [_vp, _u64, _vp, _vp, _vp, _vp, MdiscrDiscr, _vp]),
     "mdiscr_sizes_from_counts": (_int, [_vp, _vp, _u32, _u64, _vp, _vp]),
+    "mdiscr_partition_batch": (_int, [_vp, _vp, _u32, _vp]),
+    "mdiscr_server_create": (_int, [ctypes.POINTER(_vp), _vp]),
+    "mdiscr_server_destroy": (_int, [_vp]),
+    "mdiscr_server_submit": (_int, [_vp, _vp, _u64, MdiscrDiscr, ctypes.POINTER(_u64)]),
+    "mdiscr_server_poll": (_int, [_vp, _u64, ctypes.POINTER(_int)]),
+    "mdiscr_server_wait": (_int, [_vp, _u64, ctypes.POINTER(_vp), _vp,
+                                  ctypes.POINTER(ctypes.c_double)]),
+    "mdiscr_server_release": (_int, [_vp, _u64]),
+    "mdiscr_server_stats_get": (_int, [_vp, _vp]),
     "mdiscr_timing_enable": (_int, [_vp, _int]),
     "mdiscr_timing_read": (_int, [_vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                                   ctypes.POINTER(_u64)]),

@@ -15,6 +15,7 @@ SOURCES = [
     "mdiscr_api.hip",
     "soa.hip",
     "segmented.hip",
+    "serve.cpp",
 ]
 
 HIPCC_FLAGS = [
@@ -27,6 +28,7 @@ HIPCC_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
     "-Wno-unused-value",
+    "-pthread",
 ]
 
 
@@ -40,7 +42,9 @@ def _hipcc():
 def build_library(verbose=False):
     srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     tmp = LIB + ".tmp"
-    cmd = [_hipcc()] + HIPCC_FLAGS + ["-o", tmp] + srcs
+    # serve.cpp is host code; -x hip gives it the same HIP runtime headers.
+    cmd = [_hipcc()] + HIPCC_FLAGS + ["-o", tmp] + [
+        a for s in srcs for a in ((["-x", "hip", s, "-x", "none"]) if s.endswith(".cpp") else [s])]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -49,6 +53,27 @@ def build_library(verbose=False):
         raise RuntimeError("hipcc failed building libmdiscr.so")
     os.replace(tmp, LIB)
     return LIB
+
+
+HOST_PROGRAMS = {"mdiscr_segbench": "mdiscr_segbench.c"}
+BIN = os.path.join(PKG, "bin")
+
+
+def build_host_programs(verbose=False):
+    """C host drivers of the C ABI (linked against libmdiscr.so)."""
+    os.makedirs(BIN, exist_ok=True)
+    for exe, src in HOST_PROGRAMS.items():
+        cmd = ["gcc", "-O2", "-std=c11", "-D_GNU_SOURCE", "-D__HIP_PLATFORM_AMD__",
+               "-I", os.path.join(ROOT, "include"), "-I", "/opt/rocm/include",
+               "-o", os.path.join(BIN, exe), os.path.join(PKG, "host", src),
+               "-L", PKG, "-lmdiscr", "-L", "/opt/rocm/lib", "-lamdhip64", "-lm",
+               "-Wl,-rpath,$ORIGIN/..:/opt/rocm/lib"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError(f"building {exe} failed")
 
 
 def build_oracle(verbose=False):
@@ -63,6 +88,7 @@ def build_oracle(verbose=False):
 
 def build_all(verbose=False):
     build_library(verbose)
+    build_host_programs(verbose)
     build_oracle(verbose)
 
 

@@ -123,6 +123,9 @@ struct SegState {
     size_t flag_ws_bytes = 0;
     void* cnt_ws = nullptr;
     size_t cnt_ws_bytes = 0;
+    void* host_desc = nullptr;       // pinned descriptor staging (batches)
+    size_t host_desc_bytes = 0;
+    hipEvent_t staged_evt = nullptr;  // last upload out of host_desc
 };
 
 // Grow-only device buffer (the old one may still be read by queued work).
@@ -428,6 +431,8 @@ void ctx_release_extra(mdiscr_ctx* ctx) {
         for (auto& kv : ss->graphs) (void)hipGraphExecDestroy(kv.second);
         if (ss->flag_ws) (void)hipFree(ss->flag_ws);
         if (ss->cnt_ws) (void)hipFree(ss->cnt_ws);
+        if (ss->host_desc) (void)hipHostFree(ss->host_desc);
+        if (ss->staged_evt) (void)hipEventDestroy(ss->staged_evt);
         delete ss;
         ctx->extra = nullptr;
     }
@@ -528,6 +533,90 @@ mdiscr_status mdiscr_segm_flags(mdiscr_ctx* ctx, uint64_t n, const uint32_t* d_i
     }
     if (st) return st;
     return ctx_hip(ctx, hipStreamSynchronize(s), "hipStreamSynchronize");
+}
+
+mdiscr_status mdiscr_partition_batch(mdiscr_ctx* ctx, const mdiscr_request* reqs, uint32_t nreq,
+                                     void* stream) {
+    if (!ctx) return MDISCR_EINVAL;
+    if (nreq && !reqs) return ctx_fail(ctx, MDISCR_EINVAL, "null request array");
+    uint64_t total = 0;
+    uint32_t mmax = 1;
+    bool all_pow2 = true;
+    for (uint32_t i = 0; i < nreq; ++i) {
+        const mdiscr_request& r = reqs[i];
+        mdiscr_status st = check_discr(ctx, r.d);
+        if (st) return st;
+        if (r.n > 0xFFFFFFFFull) return ctx_fail(ctx, MDISCR_EINVAL, "request %u: n too large", i);
+        if (r.n && (!r.d_in || !r.d_out))
+            return ctx_fail(ctx, MDISCR_EINVAL, "request %u: null data pointer", i);
+        if (r.n && ranges_overlap(r.d_in, r.d_out, r.n * 4))
+            return ctx_fail(ctx, MDISCR_EINVAL, "request %u: output overlaps input", i);
+        total += r.n;
+        if (r.d.m > mmax) mmax = r.d.m;
+        if (r.d.kind != MDISCR_DISCR_MASK && (r.d.m & (r.d.m - 1))) all_pow2 = false;
+    }
+    if (total > MDISCR_MAX_ELEMS) return ctx_fail(ctx, MDISCR_EINVAL, "batch too large");
+    mdiscr_status st = ctx_hip(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    if (st || nreq == 0) return st;
+    hipStream_t s = (hipStream_t)stream;
+    SegState* ss = seg_state(ctx);
+    // Host descriptors in pinned staging; the previous batch's upload must
+    // have left the staging buffer before it is rewritten.
+    if (ss->staged_evt) {
+        st = ctx_hip(ctx, hipEventSynchronize(ss->staged_evt), "hipEventSynchronize");
+        if (st) return st;
+    } else {
+        st = ctx_hip(ctx, hipEventCreateWithFlags(&ss->staged_evt, hipEventDisableTiming),
+                     "hipEventCreate");
+        if (st) return st;
+    }
+    const size_t dbytes = (size_t)nreq * sizeof(SegDesc);
+    if (dbytes > ss->host_desc_bytes) {
+        if (ss->host_desc) (void)hipHostFree(ss->host_desc);
+        ss->host_desc = nullptr;
+        ss->host_desc_bytes = 0;
+        hipError_t e = hipHostMalloc(&ss->host_desc, dbytes * 2, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return ctx_fail(ctx, MDISCR_ENOMEM, "pinned staging: %s", hipGetErrorString(e));
+        }
+        ss->host_desc_bytes = dbytes * 2;
+    }
+    SegDesc* hd = static_cast<SegDesc*>(ss->host_desc);
+    for (uint32_t i = 0; i < nreq; ++i) {
+        hd[i].in = reqs[i].d_in;
+        hd[i].out = reqs[i].d_out;
+        hd[i].counts = reqs[i].d_counts;
+        hd[i].len = (uint32_t)reqs[i].n;
+        hd[i].m = reqs[i].d.m;
+        hd[i].g = mdiscr_divmagic_make(reqs[i].d.m);
+    }
+    SegWorkspace w;
+    st = seg_workspace(ctx, total, nreq, mmax, w);
+    if (st) return st;
+    st = ctx_hip(ctx, hipMemcpyAsync(w.desc, hd, dbytes, hipMemcpyHostToDevice, s), "desc upload");
+    if (st) return st;
+    st = ctx_hip(ctx, hipEventRecord(ss->staged_evt, s), "hipEventRecord");
+    if (st) return st;
+    const uint32_t nbits = class_bits(mmax);
+    int cfg = 0;
+    auto it = ss->tuned.find(bucket_key(all_pow2, nbits, total, nreq));
+    if (it != ss->tuned.end() && config_available(all_pow2, nbits, it->second)) cfg = it->second;
+    SegArgs a;
+    a.desc = w.desc;
+    a.items = w.items;
+    a.plan = w.plan;
+    a.counts = w.counts;
+    a.ld = w.ld;
+    a.nseg = nreq;
+    a.mrows = mmax;
+    a.grid = persistent_grid(ctx, cfg, 2 * (uint64_t)nreq + total / kMinTile + 2);
+    TimedLaunch tl;
+    timing_begin(ctx, s, tl);
+    const bool ok = all_pow2 ? launch_any<true>(nbits, cfg, a, s) : launch_any<false>(nbits, cfg, a, s);
+    timing_end(ctx, s, tl, "batch");
+    if (!ok) return ctx_fail(ctx, MDISCR_EUNSUPPORTED, "batch config");
+    return ctx_hip(ctx, hipGetLastError(), "batch launch");
 }
 
 }  // extern "C"
