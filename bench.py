@@ -10,7 +10,11 @@ into m = 8 classes (class = x & 7), inputs resident in HBM, through the C ABI
                 launch time (HIP events on the launch stream) vs 8 TB/s
   cpu_baseline  the reference's own SeqMdiscr (compiled from source into
                 oracle/_ref) on a bounded sample, on this host
-plus the skewed (Zipf 1.5) variant and the per-kernel breakdown.
+plus the skewed (Zipf 1.5) variant, the per-kernel breakdown, and the C host
+drivers' results: the batched segmented workload (10,000 lognormal segments,
+one graph-replayed launch vs one plain launch per segment) and the serving
+workload (Poisson arrivals; continuous-batching server vs plain per-request
+calls; p50/p99 latency and aggregate Gelem/s).
 """
 import argparse
 import json
@@ -108,6 +112,22 @@ def cpu_baseline(n_sample):
     return res
 
 
+def run_driver(args, timeout):
+    """Runs a C host driver of the C ABI and returns its JSON line (or an error)."""
+    import subprocess
+    exe = os.path.join(ROOT, "mdiscriminator_amd", "bin", args[0])
+    if not os.path.exists(exe):
+        return {"error": f"{args[0]} not built"}
+    try:
+        r = subprocess.run([exe] + [str(a) for a in args[1:]], capture_output=True, text=True,
+                           timeout=timeout)
+    except subprocess.TimeoutExpired:
+        return {"error": f"{args[0]} timed out after {timeout} s"}
+    if r.returncode != 0:
+        return {"error": f"{args[0]} exit {r.returncode}: {r.stderr[-300:]}"}
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,6 +135,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=1 << 26)
+    ap.add_argument("--no-drivers", action="store_true",
+                    help="skip the batched-segment and serving drivers")
+    ap.add_argument("--serve-requests", type=int, default=50000)
     ap.add_argument("--profile-mode", action="store_true",
                     help="for rocprofv3 counter runs: a few partition steps, then an "
                          "elementwise copy of the same 2^28 uint32 (known 1 GiB read + "
@@ -150,8 +173,9 @@ def main():
         for _ in range(args.steps):
             step()
         torch.cuda.synchronize()
+        f_in, f_out = d_in.view(torch.float32), d_out.view(torch.float32)
         for _ in range(args.steps):
-            torch.bitwise_xor(d_in, 0, out=d_out)
+            torch.neg(f_in, out=f_out)   # 1 GiB read + 1 GiB write, counter calibration
         torch.cuda.synchronize()
         log("profile mode done")
         ctx.close()
@@ -171,6 +195,13 @@ def main():
     tpath = os.path.join(ROOT, "profiles", "traffic_partition_2p28_m8.json")
     if os.path.exists(tpath):
         traffic = json.load(open(tpath)).get("bytes_per_launch")
+
+    seg = serve = None
+    if not args.no_drivers:
+        log("batched segmented vs one plain launch per segment (C driver)")
+        seg = run_driver(["mdiscr_segbench", 10000, 4096, 1.5, 4, 5], 300)
+        log(f"serving {args.serve_requests} requests (C driver)")
+        serve = run_driver(["mdiscr_servebench", args.serve_requests, 0.8, 64, 4, 16], 400)
 
     cpu = None
     if not args.no_cpu:
@@ -203,6 +234,8 @@ def main():
         "kernels_ms": {k: (round(v, 5) if v else v) for k, v in kt.items()},
         "zipf_variant": {"value": round(N * args.steps / dtz / 1e9, 3), "unit": "Gelem/s",
                          "kernels_ms": {k: (round(v, 5) if v else v) for k, v in ktz.items()}},
+        "batched_segments": seg,
+        "serving": serve,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

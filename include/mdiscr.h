@@ -173,6 +173,7 @@ typedef struct {
     uint32_t max_batch_requests; /* requests per launch */
     uint64_t max_batch_elems;    /* elements per launch */
     uint32_t max_pending;        /* submit returns MDISCR_EBUSY beyond this */
+    uint64_t reserve_bytes;      /* output arena mapped at creation (<= budget) */
 } mdiscr_server_config;
 
 typedef struct {

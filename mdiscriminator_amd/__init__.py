@@ -192,10 +192,10 @@ class Server:
     released."""
 
     def __init__(self, budget_bytes, nstreams=4, max_batch_requests=1024,
-                 max_batch_elems=1 << 28, max_pending=0, device=0):
+                 max_batch_elems=1 << 28, max_pending=0, device=0, reserve_bytes=0):
         self.lib = _lib.load()
         cfg = _lib.ServerConfig(device, budget_bytes, nstreams, max_batch_requests,
-                                max_batch_elems, max_pending)
+                                max_batch_elems, max_pending, reserve_bytes)
         h = ctypes.c_void_p()
         st = self.lib.mdiscr_server_create(ctypes.byref(h), ctypes.byref(cfg))
         if st != _lib.OK:

@@ -27,7 +27,8 @@ class MdiscrRequest(ctypes.Structure):
 class ServerConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("budget_bytes", ctypes.c_uint64),
                 ("nstreams", ctypes.c_uint32), ("max_batch_requests", ctypes.c_uint32),
-                ("max_batch_elems", ctypes.c_uint64), ("max_pending", ctypes.c_uint32)]
+                ("max_batch_elems", ctypes.c_uint64), ("max_pending", ctypes.c_uint32),
+                ("reserve_bytes", ctypes.c_uint64)]
 
 
 class ServerStats(ctypes.Structure):

@@ -55,7 +55,7 @@ def build_library(verbose=False):
     return LIB
 
 
-HOST_PROGRAMS = {"mdiscr_segbench": "mdiscr_segbench.c"}
+HOST_PROGRAMS = {"mdiscr_segbench": "mdiscr_segbench.c", "mdiscr_servebench": "mdiscr_servebench.c"}
 BIN = os.path.join(PKG, "bin")
 
 
@@ -66,7 +66,7 @@ def build_host_programs(verbose=False):
         cmd = ["gcc", "-O2", "-std=c11", "-D_GNU_SOURCE", "-D__HIP_PLATFORM_AMD__",
                "-I", os.path.join(ROOT, "include"), "-I", "/opt/rocm/include",
                "-o", os.path.join(BIN, exe), os.path.join(PKG, "host", src),
-               "-L", PKG, "-lmdiscr", "-L", "/opt/rocm/lib", "-lamdhip64", "-lm",
+               "-L", PKG, "-lmdiscr", "-L", "/opt/rocm/lib", "-lamdhip64", "-lm", "-pthread",
                "-Wl,-rpath,$ORIGIN/..:/opt/rocm/lib"]
         if verbose:
             print(" ".join(cmd), flush=True)

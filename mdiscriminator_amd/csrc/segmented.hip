@@ -124,6 +124,7 @@ struct SegState {
     void* cnt_ws = nullptr;
     size_t cnt_ws_bytes = 0;
     void* host_desc = nullptr;       // pinned descriptor staging (batches)
+    void* host_desc_dev = nullptr;   // its device-side address
     size_t host_desc_bytes = 0;
     hipEvent_t staged_evt = nullptr;  // last upload out of host_desc
 };
@@ -574,8 +575,10 @@ mdiscr_status mdiscr_partition_batch(mdiscr_ctx* ctx, const mdiscr_request* reqs
     if (dbytes > ss->host_desc_bytes) {
         if (ss->host_desc) (void)hipHostFree(ss->host_desc);
         ss->host_desc = nullptr;
+        ss->host_desc_dev = nullptr;
         ss->host_desc_bytes = 0;
-        hipError_t e = hipHostMalloc(&ss->host_desc, dbytes * 2, hipHostMallocDefault);
+        hipError_t e = hipHostMalloc(&ss->host_desc, dbytes * 2, hipHostMallocMapped);
+        if (e == hipSuccess) e = hipHostGetDevicePointer(&ss->host_desc_dev, ss->host_desc, 0);
         if (e != hipSuccess) {
             (void)hipGetLastError();
             return ctx_fail(ctx, MDISCR_ENOMEM, "pinned staging: %s", hipGetErrorString(e));
@@ -594,8 +597,8 @@ mdiscr_status mdiscr_partition_batch(mdiscr_ctx* ctx, const mdiscr_request* reqs
     SegWorkspace w;
     st = seg_workspace(ctx, total, nreq, mmax, w);
     if (st) return st;
-    st = ctx_hip(ctx, hipMemcpyAsync(w.desc, hd, dbytes, hipMemcpyHostToDevice, s), "desc upload");
-    if (st) return st;
+    hipLaunchKernelGGL(k_seg_desc_copy, dim3((nreq + 255) / 256), dim3(256), 0, s,
+                       static_cast<const SegDesc*>(ss->host_desc_dev), nreq, w.desc);
     st = ctx_hip(ctx, hipEventRecord(ss->staged_evt, s), "hipEventRecord");
     if (st) return st;
     const uint32_t nbits = class_bits(mmax);
@@ -610,7 +613,13 @@ mdiscr_status mdiscr_partition_batch(mdiscr_ctx* ctx, const mdiscr_request* reqs
     a.ld = w.ld;
     a.nseg = nreq;
     a.mrows = mmax;
-    a.grid = persistent_grid(ctx, cfg, 2 * (uint64_t)nreq + total / kMinTile + 2);
+    {   // as many workgroups as the queue will hold, up to the resident capacity
+        const uint64_t tile = (uint64_t)kConfigs[cfg].nw * kWave * kConfigs[cfg].r;
+        const uint64_t wt = (uint64_t)kWave * kConfigs[cfg].r;
+        uint64_t est = 0;
+        for (uint32_t i = 0; i < nreq; ++i) est += reqs[i].n <= wt ? 1 : (reqs[i].n + tile - 1) / tile;
+        a.grid = persistent_grid(ctx, cfg, est);
+    }
     TimedLaunch tl;
     timing_begin(ctx, s, tl);
     const bool ok = all_pow2 ? launch_any<true>(nbits, cfg, a, s) : launch_any<false>(nbits, cfg, a, s);

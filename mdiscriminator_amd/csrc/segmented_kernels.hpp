@@ -75,6 +75,13 @@ __global__ void k_seg_desc_from_offsets(const uint64_t* __restrict__ off, uint32
     desc[s] = d;
 }
 
+// Descriptors of a batch, read straight from mapped pinned host memory.
+__global__ void k_seg_desc_copy(const SegDesc* __restrict__ host, uint32_t nseg,
+                                SegDesc* __restrict__ desc) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nseg) desc[s] = host[s];
+}
+
 // ---------------------------------------------------------------------------
 // Planning: one workgroup of THREADS walks the segments in passes, with
 // block scans for the item and row counts and a max-scan for the last long

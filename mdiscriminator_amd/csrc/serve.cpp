@@ -5,8 +5,10 @@
 // pinned copy per batch), (2) admits queued requests in FIFO order while their
 // output buffers fit the budget, and (3) hands every admitted request to the
 // next free slot as one batched launch.  Output buffers come from a
-// size-class cache so steady-state admission does not call hipMalloc/hipFree
-// (hipFree would wait for the device and stall the overlapping batches).
+// size-class cache carved from 1 GiB arena chunks, so steady-state admission
+// calls neither hipMalloc nor hipFree (hipFree would wait for the device and
+// stall the overlapping batches).  Kernels write class counts straight into
+// mapped pinned memory, so a finished batch needs no copy back.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -41,8 +43,8 @@ struct Slot {
     mdiscr_ctx* ctx = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
-    uint32_t* d_counts = nullptr;  // max_batch_requests * 64
-    uint32_t* h_counts = nullptr;  // pinned mirror
+    uint32_t* h_counts = nullptr;  // mapped pinned, max_batch_requests * 64
+    uint32_t* d_counts = nullptr;  // its device-side address (kernels write here)
     std::vector<uint64_t> tickets;
     bool busy = false;
 };
@@ -70,38 +72,49 @@ struct mdiscr_server {
     bool stop = false;
     std::thread worker;
 
-    // Budget accounting: in-use plus cached bytes never exceed the budget.
+    // Output memory: size-class blocks carved from large arena chunks (one
+    // hipMalloc per chunk), recycled through per-class free lists.  A block
+    // is charged to the budget while it holds a request's output.
+    static constexpr size_t kChunk = size_t(1) << 30;
+    std::vector<std::pair<char*, size_t>> chunks;  // base, bytes used
+
     void* alloc_block(size_t cls) {
+        if (in_use + cls > cfg.budget_bytes) return nullptr;
         auto& fl = free_blocks[cls];
         if (!fl.empty()) {
             void* p = fl.back();
             fl.pop_back();
             in_use += cls;
+            peak = std::max(peak, in_use);
             return p;
         }
-        if (in_use + cls > cfg.budget_bytes) return nullptr;
-        while (held + cls > cfg.budget_bytes) {  // drop cached blocks of other sizes
-            bool dropped = false;
-            for (auto& kv : free_blocks)
-                if (!kv.second.empty()) {
-                    (void)hipFree(kv.second.back());
-                    kv.second.pop_back();
-                    held -= kv.first;
-                    dropped = true;
-                    break;
-                }
-            if (!dropped) return nullptr;
+        // first chunk with room (reserved chunks fill in order)
+        while (cur_chunk < chunks.size() && chunks[cur_chunk].second + cls > chunk_cap[cur_chunk])
+            ++cur_chunk;
+        if (cur_chunk == chunks.size()) {
+            const size_t csize = std::max(kChunk, cls);
+            if (held + csize > cfg.budget_bytes + kChunk) return nullptr;
+            if (!map_chunk(csize)) return nullptr;
         }
-        void* p = nullptr;
-        if (hipMalloc(&p, cls) != hipSuccess) {
-            (void)hipGetLastError();
-            return nullptr;
-        }
-        held += cls;
+        void* p = chunks[cur_chunk].first + chunks[cur_chunk].second;
+        chunks[cur_chunk].second += cls;
         in_use += cls;
         peak = std::max(peak, in_use);
         return p;
     }
+    size_t cur_chunk = 0;
+    bool map_chunk(size_t csize) {
+        void* p = nullptr;
+        if (hipMalloc(&p, csize) != hipSuccess) {
+            (void)hipGetLastError();
+            return false;
+        }
+        chunks.emplace_back(static_cast<char*>(p), 0);
+        chunk_cap.push_back(csize);
+        held += csize;
+        return true;
+    }
+    std::vector<size_t> chunk_cap;
     void free_block(void* p, size_t cls) {
         free_blocks[cls].push_back(p);
         in_use -= cls;
@@ -168,11 +181,6 @@ struct mdiscr_server {
                 mdiscr_status st = mdiscr_partition_batch(sl.ctx, batch.data(),
                                                           (uint32_t)batch.size(), sl.stream);
                 if (st == MDISCR_OK)
-                    st = hipMemcpyAsync(sl.h_counts, sl.d_counts,
-                                        batch.size() * MDISCR_MAX_CLASSES * 4,
-                                        hipMemcpyDeviceToHost, sl.stream) == hipSuccess
-                             ? MDISCR_OK : MDISCR_EDEVICE;
-                if (st == MDISCR_OK)
                     st = hipEventRecord(sl.done, sl.stream) == hipSuccess ? MDISCR_OK
                                                                           : MDISCR_EDEVICE;
                 if (st != MDISCR_OK) {
@@ -199,7 +207,7 @@ struct mdiscr_server {
                 for (auto& sl : slots) any_busy |= sl.busy;
                 // Poll running batches briefly; sleep until new work otherwise.
                 if (any_busy)
-                    cv_work.wait_for(lk, std::chrono::microseconds(20));
+                    cv_work.wait_for(lk, std::chrono::microseconds(2));
                 else
                     cv_work.wait_for(lk, std::chrono::milliseconds(5));
             }
@@ -228,13 +236,19 @@ mdiscr_status mdiscr_server_create(mdiscr_server** out, const mdiscr_server_conf
         const size_t cb = (size_t)cfg->max_batch_requests * MDISCR_MAX_CLASSES * 4;
         if (st != MDISCR_OK || hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess ||
-            hipMalloc((void**)&sl.d_counts, cb) != hipSuccess ||
-            hipHostMalloc((void**)&sl.h_counts, cb, hipHostMallocDefault) != hipSuccess) {
+            hipHostMalloc((void**)&sl.h_counts, cb, hipHostMallocMapped) != hipSuccess ||
+            hipHostGetDevicePointer((void**)&sl.d_counts, sl.h_counts, 0) != hipSuccess) {
             (void)hipGetLastError();
             mdiscr_server_destroy(s);
             return MDISCR_EDEVICE;
         }
     }
+    for (uint64_t r = 0; r + mdiscr_server::kChunk <= std::min(cfg->reserve_bytes, cfg->budget_bytes);
+         r += mdiscr_server::kChunk)
+        if (!s->map_chunk(mdiscr_server::kChunk)) {
+            mdiscr_server_destroy(s);
+            return MDISCR_ENOMEM;
+        }
     s->worker = std::thread([s] {
         (void)hipSetDevice(s->cfg.device);
         s->loop();
@@ -258,15 +272,11 @@ mdiscr_status mdiscr_server_destroy(mdiscr_server* s) {
     }
     (void)hipSetDevice(s->cfg.device);
     (void)hipDeviceSynchronize();
-    for (auto& kv : s->reqs)
-        if (kv.second.d_out) (void)hipFree(kv.second.d_out);
-    for (auto& kv : s->free_blocks)
-        for (void* p : kv.second) (void)hipFree(p);
+    for (auto& c : s->chunks) (void)hipFree(c.first);
     for (auto& sl : s->slots) {
         if (sl.ctx) mdiscr_ctx_destroy(sl.ctx);
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
         if (sl.done) (void)hipEventDestroy(sl.done);
-        if (sl.d_counts) (void)hipFree(sl.d_counts);
         if (sl.h_counts) (void)hipHostFree(sl.h_counts);
     }
     delete s;
