@@ -196,6 +196,29 @@ def main():
     if os.path.exists(tpath):
         traffic = json.load(open(tpath)).get("bytes_per_launch")
 
+    # Large-m struct-of-arrays records: 2^26 x (key, float32 payload bits, index).
+    soa = {}
+    if not args.no_drivers:
+        ns = 1 << 26
+        k_in = W.uniform_u31_torch(ns, SEED + 1, "cuda:0")
+        p_in = torch.randn(ns, generator=torch.Generator(device="cuda:0").manual_seed(SEED),
+                           device="cuda:0").view(torch.int32)
+        i_in = torch.arange(ns, dtype=torch.int32, device="cuda:0")
+        outs = [torch.empty_like(k_in) for _ in range(3)]
+        for m_ in (16, 32, 64):
+            def soa_step():
+                ctx.partition_soa([k_in, p_in, i_in], outs, md.Mask(m_), stream=stream)
+            dt_s = time_steps(ctx, soa_step, 10, 2)
+            kts = kernel_times(ctx, soa_step, 10, ["soa_count", "soa_scan", "soa_partition"])
+            ms_s = dt_s / 10 * 1e3
+            soa[f"m{m_}"] = {
+                "ms_per_call": round(ms_s, 4), "grec_s": round(ns / ms_s / 1e6, 2),
+                "hbm_frac_whole_path": round(ns * 24 / (ms_s * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                "partition_frac": round(ns * 24 / (kts["soa_partition"] * 1e-3) / 1e9
+                                        / PEAK_HBM_GBS, 4),
+                "kernels_ms": {k: round(v, 5) for k, v in kts.items()}}
+        del k_in, p_in, i_in, outs
+
     seg = serve = None
     if not args.no_drivers:
         log("batched segmented vs one plain launch per segment (C driver)")
@@ -234,6 +257,7 @@ def main():
         "kernels_ms": {k: (round(v, 5) if v else v) for k, v in kt.items()},
         "zipf_variant": {"value": round(N * args.steps / dtz / 1e9, 3), "unit": "Gelem/s",
                          "kernels_ms": {k: (round(v, 5) if v else v) for k, v in ktz.items()}},
+        "soa_records_2p26": soa or None,
         "batched_segments": seg,
         "serving": serve,
         "cpu_baseline": cpu,

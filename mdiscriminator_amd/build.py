@@ -76,6 +76,25 @@ def build_host_programs(verbose=False):
             raise RuntimeError(f"building {exe} failed")
 
 
+def build_test_programs(verbose=False):
+    """tests/c/mdiscr_ref_driver: the C++ mirror of the reference templates,
+    checked against the oracle (built after the oracle)."""
+    out = os.path.join(ROOT, "tests", "c", "mdiscr_ref_driver")
+    cmd = ["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__",
+           "-I", os.path.join(ROOT, "include"), "-I", os.path.join(PKG, "host"),
+           "-I", os.path.join(ROOT, "oracle"), "-I", "/opt/rocm/include",
+           "-o", out, os.path.join(ROOT, "tests", "c", "mdiscr_ref_driver.cpp"),
+           "-L", PKG, "-lmdiscr", "-L", os.path.join(ROOT, "oracle"), "-loracle",
+           "-L", "/opt/rocm/lib", "-lamdhip64",
+           "-Wl,-rpath,$ORIGIN/../../mdiscriminator_amd:$ORIGIN/../../oracle:/opt/rocm/lib"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("building tests/c/mdiscr_ref_driver failed")
+
+
 def build_oracle(verbose=False):
     r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")],
                        capture_output=True, text=True)
@@ -90,6 +109,7 @@ def build_all(verbose=False):
     build_library(verbose)
     build_host_programs(verbose)
     build_oracle(verbose)
+    build_test_programs(verbose)
 
 
 if __name__ == "__main__":
