@@ -97,4 +97,12 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nblocks) {
     return x * q + (x < r ? x : r) + i;
 }
 
+// Same XCD ranges, each walked back to front.
+__device__ __forceinline__ uint32_t xcd_tile_reversed(uint32_t b, uint32_t nblocks) {
+    const uint32_t q = nblocks / kNumXcd, r = nblocks % kNumXcd;
+    const uint32_t x = b % kNumXcd, i = b / kNumXcd;
+    const uint32_t cnt = q + (x < r ? 1u : 0u);
+    return x * q + (x < r ? x : r) + (cnt - 1u - i);
+}
+
 }  // namespace mdiscr

@@ -232,8 +232,8 @@ mdiscr_status launch_plain(mdiscr_ctx* ctx, const uint32_t* in, uint32_t* out, u
                        counts, T, ld, totals, user_counts);
     timing_end(ctx, s, tl, "tile_scan");
     timing_begin(ctx, s, tl);
-    hipLaunchKernelGGL((k_partition<kNW, kR, kCT, NBITS, kPrefetch, MDISCR_PLAIN_WB, MDISCR_PLAIN_MINW, Discr>), dim3(Kp),
-                       dim3(kNW * kWave), 0, s, in, out, n, dis, counts, totals, ld, m);
+    hipLaunchKernelGGL((k_partition<kNW, kR, kCT, NBITS, kPrefetch, MDISCR_PLAIN_WB, MDISCR_PLAIN_MINW, Discr>),
+                       dim3(Kp), dim3(kNW * kWave), 0, s, in, out, n, dis, counts, totals, ld, m);
     timing_end(ctx, s, tl, "partition");
     return ctx_hip(ctx, hipGetLastError(), "plain partition launch");
 }

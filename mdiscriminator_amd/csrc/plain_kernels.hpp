@@ -29,7 +29,41 @@
 
 #include "device_common.hpp"
 
+// Cache-policy switches for the plain kernels (1 = non-temporal hint).
+#ifndef MDISCR_NT_LOAD
+#define MDISCR_NT_LOAD 1
+#endif
+#ifndef MDISCR_NT_STORE
+#define MDISCR_NT_STORE 1
+#endif
+#ifndef MDISCR_COUNT_NT_LOAD
+#define MDISCR_COUNT_NT_LOAD 1
+#endif
+// Partition tiles in reverse order within each XCD's range: the count pass
+// reads every range front to back, so its last tiles may still sit in the
+// memory-side cache when the partition starts on them.
+#ifndef MDISCR_PLAIN_REVERSE
+#define MDISCR_PLAIN_REVERSE 0
+#endif
+
 namespace mdiscr {
+
+template <typename T>
+__device__ __forceinline__ T ld_stream(const T* p) {
+#if MDISCR_NT_LOAD
+    return __builtin_nontemporal_load(p);
+#else
+    return *p;
+#endif
+}
+template <typename T>
+__device__ __forceinline__ void st_stream(T v, T* p) {
+#if MDISCR_NT_STORE
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
 
 template <int NW, int R, int CT>
 struct Geom {
@@ -120,7 +154,13 @@ __global__ __launch_bounds__(NW* kWave) void k_chunk_count(
             const u32x4* in4 = reinterpret_cast<const u32x4*>(in + tbase);
             u32x4 q[R / 4];
 #pragma unroll
-            for (int k = 0; k < R / 4; ++k) q[k] = __builtin_nontemporal_load(in4 + k * THREADS + threadIdx.x);
+            for (int k = 0; k < R / 4; ++k) {
+#if MDISCR_COUNT_NT_LOAD
+                q[k] = __builtin_nontemporal_load(in4 + k * THREADS + threadIdx.x);
+#else
+                q[k] = in4[k * THREADS + threadIdx.x];
+#endif
+            }
 #pragma unroll
             for (int k = 0; k < R / 4; ++k) {
                 const uint32_t e[4] = {q[k].x, q[k].y, q[k].z, q[k].w};
@@ -288,8 +328,8 @@ __device__ __forceinline__ void load_rounds(const uint32_t* __restrict__ in, uin
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const uint32_t idx = wbase + r * kWave + lane;
-        if (FULL) v[r] = __builtin_nontemporal_load(in + idx);
-        else v[r] = idx < n ? __builtin_nontemporal_load(in + idx) : 0u;
+        if (FULL) v[r] = ld_stream(in + idx);
+        else v[r] = idx < n ? ld_stream(in + idx) : 0u;
     }
 }
 
@@ -313,7 +353,7 @@ __device__ __forceinline__ void write_runs(const uint32_t* stage, const uint32_t
 #pragma unroll
         for (int k = 0; k < WB; ++k) {
             const uint32_t s = (k0 + k) * THREADS + threadIdx.x;
-            if (FULL || s < valid) __builtin_nontemporal_store(x[k], out + (d[k] + s));
+            if (FULL || s < valid) st_stream(x[k], out + (d[k] + s));
         }
     }
 }
@@ -336,7 +376,11 @@ __global__ __launch_bounds__(NW* kWave, MINW) void k_partition(
 
     const uint32_t lane = lane_id();
     const uint32_t wave = threadIdx.x / kWave;
+#if MDISCR_PLAIN_REVERSE
+    const uint32_t chunk = xcd_tile_reversed(blockIdx.x, gridDim.x);
+#else
     const uint32_t chunk = xcd_tile(blockIdx.x, gridDim.x);
+#endif
     const uint32_t T = (n + TILE - 1) / TILE;
     const uint32_t t0 = chunk * CT;
     const uint32_t t1 = min(t0 + (uint32_t)CT, T);

@@ -9,8 +9,9 @@
 // output pointers, length, discriminator), so one launch can carry segments of
 // one array (offsets) or of unrelated buffers (a serving batch).  A planning
 // kernel builds the work queue on the device:
-//   W item : up to NW consecutive short segments (len <= 64*R), one per wave;
-//            each wave partitions its segment alone (no block barrier).
+//   W item : the short segments (len <= 64*R) among NW consecutive segment
+//            indices, one per wave; each wave partitions its segment alone
+//            (no block barrier).
 //   T item : tile j of a segment with len > 64*R.  A one-tile segment is
 //            written straight from LDS; tiles of longer segments take their
 //            offsets from a count pass + scan over "rows" (one row per tile
@@ -35,7 +36,7 @@ struct SegDesc {
 
 struct SegItem {
     uint32_t seg;     // (first) segment
-    uint32_t j;       // T: tile index; W: number of segments
+    uint32_t j;       // T: tile index; W: bit w = segment seg+w is short
     uint32_t ntiles;  // T: tiles of the segment (>= 1); W: 0
     uint32_t row;     // T with ntiles > 1: count row of this tile
 };
@@ -84,8 +85,7 @@ __global__ void k_seg_desc_copy(const SegDesc* __restrict__ host, uint32_t nseg,
 
 // ---------------------------------------------------------------------------
 // Planning: one workgroup of THREADS walks the segments in passes, with
-// block scans for the item and row counts and a max-scan for the last long
-// segment (which bounds each run of short segments).
+// block scans for the item and row counts.
 template <int THREADS>
 __device__ __forceinline__ uint32_t block_excl_sum(uint32_t v, uint32_t* wsum, uint32_t& total) {
     constexpr int NWAVE = THREADS / kWave;
@@ -104,67 +104,67 @@ __device__ __forceinline__ uint32_t block_excl_sum(uint32_t v, uint32_t* wsum, u
     return pre + wex;
 }
 
-template <int THREADS>
-__device__ __forceinline__ int block_incl_max(int v, int* wmax) {
-    constexpr int NWAVE = THREADS / kWave;
-    const uint32_t lane = lane_id(), wave = threadIdx.x / kWave;
-    int x = v;
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-        const int y = __shfl_up(x, d, kWave);
-        if (lane >= (uint32_t)d) x = max(x, y);
-    }
-    if (lane == kWave - 1) wmax[wave] = x;
-    __syncthreads();
-    int pre = -1;
-#pragma unroll
-    for (int w = 0; w < NWAVE; ++w)
-        if (w < (int)wave) pre = max(pre, wmax[w]);
-    __syncthreads();
-    return max(pre, x);
-}
-
 template <int THREADS, int NW, int R>
 __global__ __launch_bounds__(THREADS) void k_seg_plan(const SegDesc* __restrict__ desc,
                                                       uint32_t nseg, SegItem* __restrict__ items,
                                                       SegPlan* __restrict__ plan,
                                                       uint32_t* __restrict__ counts, uint32_t ld,
                                                       uint32_t mrows) {
+    // Segments are grouped by index, NW per group; a group's short segments
+    // (len <= 64*R) form one W item whose bit w marks segment first+w.  Each
+    // thread owns SPT consecutive segments (whole groups) per pass.
     constexpr uint32_t WT = kWave * R;          // one wave's elements
     constexpr uint32_t TILE = NW * kWave * R;   // one workgroup's elements
+    constexpr uint32_t SPT = 16;
+    static_assert(SPT % NW == 0, "whole groups per thread");
     __shared__ uint32_t wsum[THREADS / kWave];
-    __shared__ int wmax[THREADS / kWave];
     uint32_t item_carry = 0, row_carry = 0;
-    int last_long = -1;  // index of the last long segment seen so far
-    for (uint32_t base = 0; base < nseg; base += THREADS) {
-        const uint32_t s = base + threadIdx.x;
-        const bool in_range = s < nseg;
-        const uint32_t len = in_range ? desc[s].len : 0u;
-        const bool is_short = in_range && len <= WT;
-        const uint32_t tiles = (in_range && !is_short) ? (len + TILE - 1) / TILE : 0u;
-        const int lng = block_incl_max<THREADS>((in_range && !is_short) ? (int)s : -1, wmax);
-        const int run_prev = max(lng, last_long);     // last long segment at or before s
-        const bool leader = is_short && ((int)s - run_prev - 1) % NW == 0;
-        uint32_t gsz = 0;
-        if (leader) {
-            gsz = 1;
-            while (gsz < (uint32_t)NW && s + gsz < nseg && desc[s + gsz].len <= WT) ++gsz;
+    for (uint32_t base = 0; base < nseg; base += THREADS * SPT) {
+        const uint32_t s0 = base + threadIdx.x * SPT;
+        uint32_t len[SPT];
+#pragma unroll
+        for (uint32_t k = 0; k < SPT; ++k) len[k] = (s0 + k < nseg) ? desc[s0 + k].len : 0u;
+        uint32_t nitems = 0, nrows = 0;
+#pragma unroll
+        for (uint32_t g = 0; g < SPT; g += NW) {
+            bool any_short = false;
+#pragma unroll
+            for (uint32_t w = 0; w < NW; ++w) {
+                const uint32_t k = g + w;
+                if (s0 + k >= nseg) continue;
+                if (len[k] <= WT) {
+                    any_short = true;
+                } else {
+                    const uint32_t tiles = (len[k] + TILE - 1) / TILE;
+                    nitems += tiles;
+                    nrows += tiles > 1 ? tiles : 0u;
+                }
+            }
+            nitems += any_short ? 1u : 0u;
         }
-        const uint32_t nitems = tiles + (leader ? 1u : 0u);
-        const uint32_t nrows = tiles > 1 ? tiles : 0u;
         uint32_t itot, rtot;
-        const uint32_t ibase = item_carry + block_excl_sum<THREADS>(nitems, wsum, itot);
-        const uint32_t rbase = row_carry + block_excl_sum<THREADS>(nrows, wsum, rtot);
-        if (leader) items[ibase] = SegItem{s, gsz, 0u, 0u};
-        for (uint32_t j = 0; j < tiles; ++j)
-            items[ibase + j] = SegItem{s, j, tiles, tiles > 1 ? rbase + j : 0u};
+        uint32_t ib = item_carry + block_excl_sum<THREADS>(nitems, wsum, itot);
+        uint32_t rb = row_carry + block_excl_sum<THREADS>(nrows, wsum, rtot);
+#pragma unroll
+        for (uint32_t g = 0; g < SPT; g += NW) {
+            uint32_t mask = 0;
+#pragma unroll
+            for (uint32_t w = 0; w < NW; ++w)
+                if (s0 + g + w < nseg && len[g + w] <= WT) mask |= 1u << w;
+            if (mask) items[ib++] = SegItem{s0 + g, mask, 0u, 0u};
+#pragma unroll
+            for (uint32_t w = 0; w < NW; ++w) {
+                const uint32_t k = g + w;
+                if (s0 + k >= nseg || len[k] <= WT) continue;
+                const uint32_t tiles = (len[k] + TILE - 1) / TILE;
+                for (uint32_t j = 0; j < tiles; ++j)
+                    items[ib + j] = SegItem{s0 + k, j, tiles, tiles > 1 ? rb + j : 0u};
+                ib += tiles;
+                rb += tiles > 1 ? tiles : 0u;
+            }
+        }
         item_carry += itot;
         row_carry += rtot;
-        // the last long segment of this pass, for the next pass
-        if (threadIdx.x == THREADS - 1) wmax[0] = max(lng, last_long);
-        __syncthreads();
-        last_long = wmax[0];
-        __syncthreads();
     }
     if (threadIdx.x == 0) *plan = SegPlan{item_carry, row_carry};
     // A zero after the last row: the scan's exclusive prefix there is the total.
@@ -253,8 +253,9 @@ __global__ __launch_bounds__(NW* kWave) void k_seg_partition(
         const SegItem it = items[i];
         __syncthreads();  // LDS reuse across items
         if (it.ntiles == 0) {
-            // ---- W item: wave w partitions segment it.seg + w on its own.
-            if (wave < it.j) {
+            // ---- W item: wave w partitions segment it.seg + w on its own
+            //      when bit w of it.j is set.
+            if ((it.j >> wave) & 1u) {
                 const SegDesc d = desc[it.seg + wave];
                 const auto dis = SegDiscr<MASK>::make(d);
                 uint32_t v[R], packed[R];
