@@ -98,16 +98,28 @@ mdiscr_status mdiscr_u32(mdiscr_ctx* ctx, uint64_t n, uint32_t num_hwd_thds,
 
 /* ---- struct-of-arrays partition (MdiscrHost_tuples.cu.h:262-429) ------ */
 
-/* Partitions ncols parallel columns by the class of the 32-bit key column
+/* Partitions ncols parallel columns by the class of the key column
  * d_cols[key_col]; every column moves to the same index as raw bits.
  * d_cols / d_out_cols / col_bytes are HOST arrays of ncols entries holding
- * device pointers / element sizes in {1,2,4,8}.  ncols <= 8. */
+ * device pointers / element sizes in {1,2,4,8}.  ncols <= 8.  The key column
+ * holds 32-bit keys (class = D(key), like ModTuple<N>) or 1-byte values
+ * (class = D(value): a caller-computed class array with m > value). */
 mdiscr_status mdiscr_partition_soa(mdiscr_ctx* ctx, uint32_t ncols,
                                    const void* const* d_cols,
                                    const uint32_t* col_bytes, uint32_t key_col,
                                    void* const* d_out_cols, uint64_t n,
                                    mdiscr_discr d, uint32_t* d_counts,
                                    void* stream);
+
+/* ---- array-valued elements (MdiscrHost_arrays.cu.h:250-470) ------------ */
+
+/* Elements are len consecutive uint32 words (d_in[i*len .. i*len+len)); the
+ * class is D(word 0) (ModArray<N,LEN>, MdiscrHost_arrays.cu.h:27-36) and each
+ * whole element moves.  1 <= len <= 8; n counts elements. */
+mdiscr_status mdiscr_partition_arrays(mdiscr_ctx* ctx, const uint32_t* d_in,
+                                      uint32_t* d_out, uint64_t n, uint32_t len,
+                                      mdiscr_discr d, uint32_t* d_counts,
+                                      void* stream);
 
 /* ---- segmented partition (MdiscrSegmHost.cu.h:36-230) ---------------- */
 

@@ -135,6 +135,15 @@ class Context:
                                            _ptr(d_counts), _stream(stream))
         self._check(st, "mdiscr_partition_soa")
 
+    # -- array-valued elements --------------------------------------------------
+    def partition_arrays(self, d_in, d_out, length, discr, d_counts=None, n=None, stream=None):
+        """Elements of `length` uint32 words stored flat; class from word 0
+        (MdiscrHost_arrays.cu.h:14-96)."""
+        n = d_in.numel() // length if n is None else n
+        st = self.lib.mdiscr_partition_arrays(self.h, _ptr(d_in), _ptr(d_out), n, length,
+                                              discr.c(), _ptr(d_counts), _stream(stream))
+        self._check(st, "mdiscr_partition_arrays")
+
     # -- segmented -------------------------------------------------------------
     def partition_segmented(self, d_in, d_out, d_seg_off, discr, h_seg_off=None,
                             d_seg_counts=None, stream=None):

@@ -40,17 +40,39 @@ def _hipcc():
 
 
 def build_library(verbose=False):
+    """Compiles every source to an object in parallel, then links the library."""
     srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    objdir = os.path.join(PKG, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    compile_flags = [f for f in HIPCC_FLAGS if f != "-shared"]
+    procs = []
+    objs = []
+    for src in srcs:
+        obj = os.path.join(objdir, os.path.basename(src) + ".o")
+        objs.append(obj)
+        # serve.cpp is host code; -x hip gives it the same HIP runtime headers.
+        lang = ["-x", "hip"] if src.endswith(".cpp") else []
+        cmd = [_hipcc()] + compile_flags + ["-c"] + lang + [src, "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                                            text=True)))
+    failed = False
+    for cmd, p in procs:
+        out, err = p.communicate()
+        if p.returncode != 0:
+            sys.stderr.write(out + err)
+            failed = True
+    if failed:
+        raise RuntimeError("hipcc failed building libmdiscr.so")
     tmp = LIB + ".tmp"
-    # serve.cpp is host code; -x hip gives it the same HIP runtime headers.
-    cmd = [_hipcc()] + HIPCC_FLAGS + ["-o", tmp] + [
-        a for s in srcs for a in ((["-x", "hip", s, "-x", "none"]) if s.endswith(".cpp") else [s])]
+    cmd = [_hipcc()] + HIPCC_FLAGS + ["-o", tmp] + objs
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building libmdiscr.so")
+        raise RuntimeError("linking libmdiscr.so failed")
     os.replace(tmp, LIB)
     return LIB
 

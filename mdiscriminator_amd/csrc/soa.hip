@@ -12,57 +12,127 @@ using namespace mdiscr;
 
 namespace {
 
-constexpr int kNW = 8;
-constexpr int kR = 8;
+// Tile geometry by staging width and class bits.  8-byte staging (a column
+// wider than 4 bytes): 8 waves x 8 rounds = 4096 elements.  4-byte staging:
+// the same 4096-element tile up to 32 classes, and 16 waves x 8 rounds = 8192
+// elements for 64 classes, where the longer class runs pay for the bigger
+// workgroup (measured on 2^26 three-column records).
 constexpr int kCT = 1;    // tiles per partition workgroup
 constexpr int kCCT = 8;   // tiles per count workgroup
-using SoaGeom = Geom<kNW, kR, kCT>;
-constexpr uint32_t kTile = SoaGeom::kTile;
 
-template <int NBITS, class Discr>
+template <typename ST, int NBITS>
+struct SoaGeom {
+    static constexpr int NW = (sizeof(ST) == 4 && NBITS >= 6) ? 16 : 8;
+    static constexpr int R = 8;
+    static constexpr uint32_t TILE = NW * kWave * R;
+};
+
+uint32_t soa_tile(bool wide, uint32_t nbits) {
+    return nbits >= 6 && !wide ? SoaGeom<uint32_t, 6>::TILE : SoaGeom<uint64_t, 0>::TILE;
+}
+
+template <int NBITS, typename KT, typename ST, class Discr>
 mdiscr_status launch_soa(mdiscr_ctx* ctx, const SoaCols& cols, uint32_t n, Discr dis,
                          uint32_t m, uint32_t* user_counts, hipStream_t s, uint32_t* ws) {
-    const uint32_t T = (uint32_t)(((uint64_t)n + kTile - 1) / kTile);
+    constexpr int NW = SoaGeom<ST, NBITS>::NW, R = SoaGeom<ST, NBITS>::R;
+    constexpr uint32_t TILE = SoaGeom<ST, NBITS>::TILE;
+    const uint32_t T = (uint32_t)(((uint64_t)n + TILE - 1) / TILE);
     const uint32_t ld = (T + 3u) & ~3u;
     const uint32_t Kc = (T + kCCT - 1) / kCCT;
     const uint32_t Kp = (T + kCT - 1) / kCT;
     uint32_t* counts = ws;
     uint32_t* totals = ws + (size_t)m * ld;
-    const uint32_t* key = static_cast<const uint32_t*>(cols.in[cols.key]);
-    const bool vec = ((uintptr_t)key & 15u) == 0;
+    const KT* key = static_cast<const KT*>(cols.in[cols.key]);
+    const uint32_t kstride = cols.stride[cols.key];
     TimedLaunch tl;
     timing_begin(ctx, s, tl);
-    if (vec)
-        hipLaunchKernelGGL((k_chunk_count<kNW, kR, kCCT, NBITS, true, Discr>), dim3(Kc),
-                           dim3(kNW * kWave), 0, s, key, n, dis, counts, ld, m);
+    if (sizeof(KT) == 4 && kstride == 1 && ((uintptr_t)key & 15u) == 0)
+        hipLaunchKernelGGL((k_chunk_count<NW, R, kCCT, NBITS, true, Discr>), dim3(Kc),
+                           dim3(NW * kWave), 0, s, reinterpret_cast<const uint32_t*>(key), n, dis,
+                           counts, ld, m);
     else
-        hipLaunchKernelGGL((k_chunk_count<kNW, kR, kCCT, NBITS, false, Discr>), dim3(Kc),
-                           dim3(kNW * kWave), 0, s, key, n, dis, counts, ld, m);
+        hipLaunchKernelGGL((k_key_count<NW, R, kCCT, NBITS, KT, Discr>), dim3(Kc), dim3(NW * kWave),
+                           0, s, key, kstride, n, dis, counts, ld, m);
     timing_end(ctx, s, tl, "soa_count");
     timing_begin(ctx, s, tl);
     hipLaunchKernelGGL((k_tile_scan<1024, 64>), dim3(m), dim3(1024), 0, s, counts, T, ld, totals,
-                       user_counts);
+                       user_counts, (const uint32_t*)nullptr);
     timing_end(ctx, s, tl, "soa_scan");
     timing_begin(ctx, s, tl);
-    hipLaunchKernelGGL((k_partition_soa<kNW, kR, kCT, NBITS, Discr>), dim3(Kp), dim3(kNW * kWave),
-                       0, s, cols, n, dis, counts, totals, ld, m);
+    hipLaunchKernelGGL((k_partition_soa<NW, R, kCT, NBITS, KT, ST, Discr>), dim3(Kp),
+                       dim3(NW * kWave), 0, s, cols, n, dis, counts, totals, ld, m);
     timing_end(ctx, s, tl, "soa_partition");
     return ctx_hip(ctx, hipGetLastError(), "SoA partition launch");
 }
 
-template <class Discr>
+template <typename KT, typename ST, class Discr>
 mdiscr_status dispatch_soa(mdiscr_ctx* ctx, const SoaCols& cols, uint32_t n, Discr dis,
                            uint32_t m, uint32_t* user_counts, hipStream_t s, uint32_t* ws) {
     switch (class_bits(m)) {
-        case 0: return launch_soa<0>(ctx, cols, n, dis, m, user_counts, s, ws);
-        case 1: return launch_soa<1>(ctx, cols, n, dis, m, user_counts, s, ws);
-        case 2: return launch_soa<2>(ctx, cols, n, dis, m, user_counts, s, ws);
-        case 3: return launch_soa<3>(ctx, cols, n, dis, m, user_counts, s, ws);
-        case 4: return launch_soa<4>(ctx, cols, n, dis, m, user_counts, s, ws);
-        case 5: return launch_soa<5>(ctx, cols, n, dis, m, user_counts, s, ws);
-        case 6: return launch_soa<6>(ctx, cols, n, dis, m, user_counts, s, ws);
+        case 0: return launch_soa<0, KT, ST>(ctx, cols, n, dis, m, user_counts, s, ws);
+        case 1: return launch_soa<1, KT, ST>(ctx, cols, n, dis, m, user_counts, s, ws);
+        case 2: return launch_soa<2, KT, ST>(ctx, cols, n, dis, m, user_counts, s, ws);
+        case 3: return launch_soa<3, KT, ST>(ctx, cols, n, dis, m, user_counts, s, ws);
+        case 4: return launch_soa<4, KT, ST>(ctx, cols, n, dis, m, user_counts, s, ws);
+        case 5: return launch_soa<5, KT, ST>(ctx, cols, n, dis, m, user_counts, s, ws);
+        case 6: return launch_soa<6, KT, ST>(ctx, cols, n, dis, m, user_counts, s, ws);
     }
     return ctx_fail(ctx, MDISCR_EUNSUPPORTED, "m=%u", m);
+}
+
+template <typename KT, typename ST>
+mdiscr_status dispatch_discr(mdiscr_ctx* ctx, const SoaCols& cols, uint32_t n, mdiscr_discr d,
+                             uint32_t* user_counts, hipStream_t s, uint32_t* ws) {
+    if (d.kind == MDISCR_DISCR_MASK || (d.m & (d.m - 1)) == 0)
+        return dispatch_soa<KT, ST>(ctx, cols, n, DiscrMask{d.m - 1}, d.m, user_counts, s, ws);
+    return dispatch_soa<KT, ST>(ctx, cols, n, DiscrMod{mdiscr_divmagic_make(d.m)}, d.m,
+                                user_counts, s, ws);
+}
+
+// Validated columns -> launch (shared by the SoA and array entry points).
+mdiscr_status run_soa(mdiscr_ctx* ctx, SoaCols& cols, uint64_t n, mdiscr_discr d,
+                      uint32_t* d_counts, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    mdiscr_status st = ctx_hip(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    if (st) return st;
+    if (n == 0) {
+        if (d_counts) {
+            hipLaunchKernelGGL(k_fill_u32<>, dim3(1), dim3(64), 0, s, d_counts, 0u, d.m);
+            return ctx_hip(ctx, hipGetLastError(), "fill launch");
+        }
+        return MDISCR_OK;
+    }
+    bool wide = false;
+    for (uint32_t j = 0; j < cols.ncols; ++j) wide |= cols.bytes[j] > 4;
+    const uint64_t tile = soa_tile(wide, class_bits(d.m));
+    const uint64_t T = (n + tile - 1) / tile;
+    const uint64_t ld = (T + 3) & ~3ull;
+    void* ws = nullptr;
+    st = ctx_workspace(ctx, (ld + 1) * d.m * sizeof(uint32_t), &ws);
+    if (st) return st;
+    uint32_t* w = static_cast<uint32_t*>(ws);
+    const bool key8 = cols.bytes[cols.key] == 1;
+    if (wide)
+        return key8 ? dispatch_discr<uint8_t, uint64_t>(ctx, cols, (uint32_t)n, d, d_counts, s, w)
+                    : dispatch_discr<uint32_t, uint64_t>(ctx, cols, (uint32_t)n, d, d_counts, s, w);
+    return key8 ? dispatch_discr<uint8_t, uint32_t>(ctx, cols, (uint32_t)n, d, d_counts, s, w)
+                : dispatch_discr<uint32_t, uint32_t>(ctx, cols, (uint32_t)n, d, d_counts, s, w);
+}
+
+// Input and output extents must not overlap, column against column.
+mdiscr_status check_overlaps(mdiscr_ctx* ctx, const SoaCols& cols, uint64_t n) {
+    for (uint32_t j = 0; j < cols.ncols; ++j) {
+        const uint64_t ej = n ? ((n - 1) * cols.stride[j] + 1) * cols.bytes[j] : 0;
+        for (uint32_t i = 0; i < cols.ncols; ++i) {
+            const uint64_t ei = n ? ((n - 1) * cols.stride[i] + 1) * cols.bytes[i] : 0;
+            if (extents_overlap(cols.out[j], ej, cols.in[i], ei))
+                return ctx_fail(ctx, MDISCR_EINVAL, "output column %u overlaps an input", j);
+            if (i != j && cols.stride[i] == 1 && cols.stride[j] == 1 &&
+                extents_overlap(cols.out[j], ej, cols.out[i], ei))
+                return ctx_fail(ctx, MDISCR_EINVAL, "output columns %u and %u overlap", j, i);
+        }
+    }
+    return MDISCR_OK;
 }
 
 }  // namespace
@@ -78,8 +148,8 @@ extern "C" mdiscr_status mdiscr_partition_soa(mdiscr_ctx* ctx, uint32_t ncols,
     if (st) return st;
     if (ncols == 0 || ncols > (uint32_t)kMaxCols || !d_cols || !col_bytes || !d_out_cols)
         return ctx_fail(ctx, MDISCR_EINVAL, "need 1..%d columns", kMaxCols);
-    if (key_col >= ncols || col_bytes[key_col] != 4)
-        return ctx_fail(ctx, MDISCR_EINVAL, "key column must be a 32-bit column");
+    if (key_col >= ncols || (col_bytes[key_col] != 4 && col_bytes[key_col] != 1))
+        return ctx_fail(ctx, MDISCR_EINVAL, "key column must hold 4-byte keys or 1-byte classes");
     if (n > MDISCR_MAX_ELEMS) return ctx_fail(ctx, MDISCR_EINVAL, "n too large");
     SoaCols cols = {};
     cols.ncols = ncols;
@@ -95,34 +165,34 @@ extern "C" mdiscr_status mdiscr_partition_soa(mdiscr_ctx* ctx, uint32_t ncols,
         cols.in[j] = d_cols[j];
         cols.out[j] = d_out_cols[j];
         cols.bytes[j] = b;
+        cols.stride[j] = 1;
     }
-    for (uint32_t j = 0; j < ncols; ++j) {
-        const uint64_t ej = n * cols.bytes[j];
-        for (uint32_t i = 0; i < ncols; ++i) {
-            const uint64_t ei = n * cols.bytes[i];
-            if (extents_overlap(cols.out[j], ej, cols.in[i], ei) ||
-                (i != j && extents_overlap(cols.out[j], ej, cols.out[i], ei)))
-                return ctx_fail(ctx, MDISCR_EINVAL, "output column %u overlaps a column", j);
-        }
-    }
-    hipStream_t s = (hipStream_t)stream;
-    st = ctx_hip(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    st = check_overlaps(ctx, cols, n);
     if (st) return st;
-    if (n == 0) {
-        if (d_counts) {
-            hipLaunchKernelGGL(k_fill_u32<>, dim3(1), dim3(64), 0, s, d_counts, 0u, d.m);
-            return ctx_hip(ctx, hipGetLastError(), "fill launch");
-        }
-        return MDISCR_OK;
-    }
-    const uint64_t T = (n + kTile - 1) / kTile;
-    const uint64_t ld = (T + 3) & ~3ull;
-    void* ws = nullptr;
-    st = ctx_workspace(ctx, (ld + 1) * d.m * sizeof(uint32_t), &ws);
+    return run_soa(ctx, cols, n, d, d_counts, stream);
+}
+
+extern "C" mdiscr_status mdiscr_partition_arrays(mdiscr_ctx* ctx, const uint32_t* d_in,
+                                                 uint32_t* d_out, uint64_t n, uint32_t len,
+                                                 mdiscr_discr d, uint32_t* d_counts,
+                                                 void* stream) {
+    if (!ctx) return MDISCR_EINVAL;
+    mdiscr_status st = check_discr(ctx, d);
     if (st) return st;
-    if (d.kind == MDISCR_DISCR_MASK || (d.m & (d.m - 1)) == 0)
-        return dispatch_soa(ctx, cols, (uint32_t)n, DiscrMask{d.m - 1}, d.m, d_counts, s,
-                            (uint32_t*)ws);
-    return dispatch_soa(ctx, cols, (uint32_t)n, DiscrMod{mdiscr_divmagic_make(d.m)}, d.m,
-                        d_counts, s, (uint32_t*)ws);
+    if (len == 0 || len > (uint32_t)kMaxCols)
+        return ctx_fail(ctx, MDISCR_EINVAL, "array length %u outside [1,%d]", len, kMaxCols);
+    if (n > MDISCR_MAX_ELEMS) return ctx_fail(ctx, MDISCR_EINVAL, "n too large");
+    if (n && (!d_in || !d_out)) return ctx_fail(ctx, MDISCR_EINVAL, "null data pointer");
+    if (n && ranges_overlap(d_in, d_out, n * len * 4))
+        return ctx_fail(ctx, MDISCR_EINVAL, "output overlaps input");
+    SoaCols cols = {};
+    cols.ncols = len;
+    cols.key = 0;  // class from element word 0 (ModArray, MdiscrHost_arrays.cu.h:27-36)
+    for (uint32_t j = 0; j < len; ++j) {
+        cols.in[j] = d_in + j;
+        cols.out[j] = d_out + j;
+        cols.bytes[j] = 4;
+        cols.stride[j] = len;
+    }
+    return run_soa(ctx, cols, n, d, d_counts, stream);
 }

@@ -1,16 +1,21 @@
 // soa_kernels.hpp -- struct-of-arrays m-way stable partition on gfx950.
 //
 // Replaces the reference's three-column tuple path (MdiscrHost_tuples.cu.h:
-// 262-429, MdiscrKernels_tuples.cu.h:4-236): the class comes from one 32-bit
-// key column (ModTuple<N>::apply, MdiscrHost_tuples.cu.h:44-45) and every
-// column moves to the same destination, as raw bits.
+// 262-429, MdiscrKernels_tuples.cu.h:4-236): the class comes from one key
+// column (ModTuple<N>::apply, MdiscrHost_tuples.cu.h:44-45) and every column
+// moves to the same destination, as raw bits.  Columns carry an element
+// stride, so fixed-length array elements stored flat (the reference's array
+// variant, MdiscrHost_arrays.cu.h:14-96: class from element word 0, the whole
+// element moved) are LEN strided 32-bit columns with word 0 as the key.  A
+// 1-byte key column lets a caller-computed class array be the discriminator.
 //
-// Same three launches as the plain path; the count pass is the plain
-// k_chunk_count over the key column.  The partition ranks the key exactly as
-// the plain kernel does, records each element's class in LDS beside its slot,
-// then moves the columns one at a time through one LDS staging buffer:
-// load (same indices as the keys) -> stage at the element's slot -> write
-// contiguous class runs.  Bytes per record: 4 (count) + 2 * sum(col bytes).
+// Same three launches as the plain path; the count pass reads only the key
+// column.  The partition ranks the key exactly as the plain kernel does,
+// records each element's class in LDS beside its slot, then moves the columns
+// one at a time through one LDS staging buffer of element type ST (4 bytes
+// when no column is wider, so larger tiles fit): load (same indices as the
+// keys) -> stage at the element's slot -> write contiguous class runs.
+// Bytes per record: key bytes (count) + 2 * sum(column bytes).
 #pragma once
 
 #include "plain_kernels.hpp"
@@ -23,16 +28,80 @@ struct SoaCols {
     const void* in[kMaxCols];
     void* out[kMaxCols];
     uint32_t bytes[kMaxCols];
+    uint32_t stride[kMaxCols];  // element stride, in elements of that column
     uint32_t ncols;
     uint32_t key;
 };
 
+template <typename T>
+__device__ __forceinline__ T ld_nt(const T* p) {
+    return __builtin_nontemporal_load(p);
+}
+
+// ---------------------------------------------------------------------------
+// count pass over a (possibly strided, possibly 1-byte) key column.
+template <int NW, int R, int CT, int NBITS, typename KT, class Discr>
+__global__ __launch_bounds__(NW* kWave) void k_key_count(
+    const KT* __restrict__ key, uint32_t kstride, uint32_t n, Discr dis,
+    uint32_t* __restrict__ counts, uint32_t ld, uint32_t m) {
+    constexpr int MC = 1 << NBITS;
+    constexpr int TILE = NW * kWave * R;
+    constexpr int THREADS = NW * kWave;
+    constexpr bool PACKED = LaneCounts<NBITS>::kPacked;
+    static_assert(R <= 255, "byte counters per lane");
+    __shared__ uint32_t hist[2][NW][MC];
+    const uint32_t lane = lane_id();
+    const uint32_t wave = threadIdx.x / kWave;
+    const uint32_t chunk = xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t T = (n + TILE - 1) / TILE;
+    const uint32_t t0 = chunk * CT;
+    const uint32_t t1 = min(t0 + (uint32_t)CT, T);
+    for (uint32_t t = t0; t < t1; ++t) {
+        uint32_t (*h)[MC] = hist[(t - t0) & 1u];
+        if (!PACKED) {
+            for (int c = lane; c < MC; c += kWave) h[wave][c] = 0;
+            __builtin_amdgcn_wave_barrier();
+        }
+        LaneCounts<NBITS> lc;
+        const uint32_t tbase = t * (uint32_t)TILE;
+        const bool full = (uint64_t)tbase + TILE <= n;
+        uint32_t v[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const uint32_t idx = tbase + k * THREADS + threadIdx.x;
+            v[k] = (full || idx < n) ? (uint32_t)ld_nt(key + (size_t)idx * kstride) : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const uint32_t idx = tbase + k * THREADS + threadIdx.x;
+            if (full || idx < n) {
+                const uint32_t c = dis(v[k]);
+                if (PACKED) lc.add(c);
+                else atomicAdd(&h[wave][c], 1u);
+            }
+        }
+        if (PACKED) {
+            const uint32_t cnt = lc.wave_class_count(lane);
+            if (lane < (uint32_t)MC) h[wave][lane] = cnt;
+        }
+        __syncthreads();
+        for (uint32_t c = threadIdx.x; c < m; c += THREADS) {
+            uint32_t s = 0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) s += h[w][c];
+            counts[(size_t)c * ld + t] = s;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Moves one column of the current tile: load at the tile's element indices,
 // stage at each element's LDS slot, write slot s to out[dest[s]].
 template <typename T, int NW, int R>
-__device__ __forceinline__ void soa_move_column(const void* __restrict__ src_v,
-                                                void* __restrict__ dst_v, uint32_t wbase,
-                                                uint32_t n, bool full, const uint32_t (&slot)[R],
+__device__ __forceinline__ void soa_move_column(const void* __restrict__ src_v, uint32_t sstride,
+                                                void* __restrict__ dst_v, uint32_t dstride,
+                                                uint32_t wbase, uint32_t n, bool full,
+                                                const uint32_t (&slot)[R],
                                                 const uint32_t (&dest)[R], uint32_t valid,
                                                 void* stage_v, uint32_t lane) {
     constexpr int THREADS = NW * kWave;
@@ -43,7 +112,7 @@ __device__ __forceinline__ void soa_move_column(const void* __restrict__ src_v,
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const uint32_t idx = wbase + r * kWave + lane;
-        x[r] = (full || idx < n) ? __builtin_nontemporal_load(src + idx) : T(0);
+        x[r] = (full || idx < n) ? ld_nt(src + (size_t)idx * sstride) : T(0);
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) stage[slot[r]] = x[r];
@@ -51,44 +120,45 @@ __device__ __forceinline__ void soa_move_column(const void* __restrict__ src_v,
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         const uint32_t s = k * THREADS + threadIdx.x;
-        if (s < valid) __builtin_nontemporal_store(stage[s], dst + dest[k]);
+        if (s < valid) __builtin_nontemporal_store(stage[s], dst + (size_t)dest[k] * dstride);
     }
     __syncthreads();
 }
 
 // The key column is already in registers: stage and write it without a reload.
-template <int NW, int R>
+template <typename KT, int NW, int R>
 __device__ __forceinline__ void soa_write_keys(const uint32_t (&v)[R], void* __restrict__ dst_v,
-                                               const uint32_t (&slot)[R],
+                                               uint32_t dstride, const uint32_t (&slot)[R],
                                                const uint32_t (&dest)[R], uint32_t valid,
-                                               uint32_t* stage) {
+                                               void* stage_v) {
     constexpr int THREADS = NW * kWave;
-    uint32_t* __restrict__ dst = static_cast<uint32_t*>(dst_v);
+    KT* __restrict__ dst = static_cast<KT*>(dst_v);
+    KT* stage = static_cast<KT*>(stage_v);
 #pragma unroll
-    for (int r = 0; r < R; ++r) stage[slot[r]] = v[r];
+    for (int r = 0; r < R; ++r) stage[slot[r]] = (KT)v[r];
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         const uint32_t s = k * THREADS + threadIdx.x;
-        if (s < valid) __builtin_nontemporal_store(stage[s], dst + dest[k]);
+        if (s < valid) __builtin_nontemporal_store(stage[s], dst + (size_t)dest[k] * dstride);
     }
     __syncthreads();
 }
 
-template <int NW, int R, int CT, int NBITS, class Discr>
+template <int NW, int R, int CT, int NBITS, typename KT, typename ST, class Discr>
 __global__ __launch_bounds__(NW* kWave) void k_partition_soa(
     SoaCols cols, uint32_t n, Discr dis, const uint32_t* __restrict__ prefix,
     const uint32_t* __restrict__ totals, uint32_t ld, uint32_t m) {
-    using G = Geom<NW, R, CT>;
     constexpr int MC = 1 << NBITS;
-    constexpr int TILE = G::kTile;
-    constexpr int THREADS = G::kThreads;
-    __shared__ uint64_t stage[TILE];     // widest column element
+    constexpr int TILE = NW * kWave * R;
+    constexpr int THREADS = NW * kWave;
+    __shared__ ST stage[TILE];           // widest column element
     __shared__ uint8_t cls[TILE];        // class of each LDS slot
     __shared__ uint32_t cursor[NW][MC];
     __shared__ uint32_t dpos[MC];
 
-    const uint32_t* __restrict__ key = static_cast<const uint32_t*>(cols.in[cols.key]);
+    const KT* __restrict__ key = static_cast<const KT*>(cols.in[cols.key]);
+    const uint32_t kstride = cols.stride[cols.key];
     const uint32_t lane = lane_id();
     const uint32_t wave = threadIdx.x / kWave;
     const uint32_t chunk = xcd_tile(blockIdx.x, gridDim.x);
@@ -110,8 +180,11 @@ __global__ __launch_bounds__(NW* kWave) void k_partition_soa(
         const uint32_t valid = full ? (uint32_t)TILE : n - tbase;
 
         uint32_t v[R];
-        if (full) load_rounds<R, true>(key, v, wbase, n, lane);
-        else load_rounds<R, false>(key, v, wbase, n, lane);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const uint32_t idx = wbase + r * kWave + lane;
+            v[r] = (full || idx < n) ? (uint32_t)ld_nt(key + (size_t)idx * kstride) : 0u;
+        }
         for (int c = lane; c < MC; c += kWave) cursor[wave][c] = 0;
         __builtin_amdgcn_wave_barrier();
 
@@ -146,26 +219,27 @@ __global__ __launch_bounds__(NW* kWave) void k_partition_soa(
         // 4. Every column through the staging buffer.
         for (uint32_t j = 0; j < cols.ncols; ++j) {
             if (j == cols.key) {
-                soa_write_keys<NW, R>(v, cols.out[j], slot, dest, valid,
-                                      reinterpret_cast<uint32_t*>(stage));
+                soa_write_keys<KT, NW, R>(v, cols.out[j], cols.stride[j], slot, dest, valid, stage);
                 continue;
             }
+            const uint32_t ss = cols.stride[j];
             switch (cols.bytes[j]) {
                 case 1:
-                    soa_move_column<uint8_t, NW, R>(cols.in[j], cols.out[j], wbase, n, full, slot,
-                                                    dest, valid, stage, lane);
+                    soa_move_column<uint8_t, NW, R>(cols.in[j], ss, cols.out[j], ss, wbase, n, full,
+                                                    slot, dest, valid, stage, lane);
                     break;
                 case 2:
-                    soa_move_column<uint16_t, NW, R>(cols.in[j], cols.out[j], wbase, n, full,
-                                                     slot, dest, valid, stage, lane);
+                    soa_move_column<uint16_t, NW, R>(cols.in[j], ss, cols.out[j], ss, wbase, n,
+                                                     full, slot, dest, valid, stage, lane);
                     break;
                 case 4:
-                    soa_move_column<uint32_t, NW, R>(cols.in[j], cols.out[j], wbase, n, full,
-                                                     slot, dest, valid, stage, lane);
+                    soa_move_column<uint32_t, NW, R>(cols.in[j], ss, cols.out[j], ss, wbase, n,
+                                                     full, slot, dest, valid, stage, lane);
                     break;
                 default:
-                    soa_move_column<uint64_t, NW, R>(cols.in[j], cols.out[j], wbase, n, full,
-                                                     slot, dest, valid, stage, lane);
+                    if constexpr (sizeof(ST) >= 8)
+                        soa_move_column<uint64_t, NW, R>(cols.in[j], ss, cols.out[j], ss, wbase,
+                                                         n, full, slot, dest, valid, stage, lane);
                     break;
             }
         }

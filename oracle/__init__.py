@@ -35,6 +35,7 @@ def lib():
         L = ctypes.CDLL(ORACLE_SO)
         L.oracle_seq_mdiscr_u32.argtypes = [_u64, _vp, _vp, _vp, _u32, _u32, _vp]
         L.oracle_seq_mdiscr_soa.argtypes = [_u64, _u32, _vp, _vp, _u32, _vp, _u32, _u32, _vp]
+        L.oracle_seq_mdiscr_arrays.argtypes = [_u64, _u32, _vp, _vp, _u32, _u32, _vp]
         L.oracle_seq_mdiscr_segm.argtypes = [_u64, _vp, _vp, _u64, _vp, _u32, _u32, _vp]
         L.oracle_segm_flags.argtypes = [_u64, _vp, _vp, _vp, _vp, _u32, _u32]
         L.oracle_sizes_from_counts.argtypes = [_vp, _u32, _u64, _vp]
@@ -88,6 +89,17 @@ def seq_mdiscr_soa(cols, m, key_col=0, kind=MOD):
     _check(lib().oracle_seq_mdiscr_soa(n, k, pin, nb, key_col, pout, kind, m, _p(cnt)),
            "oracle_seq_mdiscr_soa")
     return outs, cnt
+
+
+def seq_mdiscr_arrays(x, length, m, kind=MOD):
+    """x: uint32 array of n*length words -> (out, counts)."""
+    x = np.ascontiguousarray(x, dtype=np.uint32)
+    n = len(x) // length
+    out = np.empty_like(x)
+    cnt = np.zeros(m, dtype=np.uint32)
+    _check(lib().oracle_seq_mdiscr_arrays(n, length, _p(x), _p(out), kind, m, _p(cnt)),
+           "oracle_seq_mdiscr_arrays")
+    return out, cnt
 
 
 def seq_mdiscr_segm(x, seg_off, m, kind=MOD):

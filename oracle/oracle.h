@@ -33,13 +33,19 @@ int oracle_seq_mdiscr_u32(uint64_t n, const uint32_t* in, uint32_t* out,
                           uint32_t* sizes, uint32_t kind, uint32_t m,
                           uint32_t* counts);
 
-/* Struct-of-arrays partition: the class comes from the 32-bit key column
- * cols[key_col]; every column is moved to the same destination index as raw
- * bytes (col_bytes[j] in {1,2,4,8}). */
+/* Struct-of-arrays partition: the class comes from the key column
+ * cols[key_col] (32-bit keys, or 1-byte values); every column is moved to the
+ * same destination index as raw bytes (col_bytes[j] in {1,2,4,8}). */
 int oracle_seq_mdiscr_soa(uint64_t n, uint32_t ncols, const void* const* cols,
                           const uint32_t* col_bytes, uint32_t key_col,
                           void* const* out_cols, uint32_t kind, uint32_t m,
                           uint32_t* counts);
+
+/* Array-valued elements: element i is in[i*len .. i*len+len), classified by
+ * word 0 and moved whole. */
+int oracle_seq_mdiscr_arrays(uint64_t n, uint32_t len, const uint32_t* in,
+                             uint32_t* out, uint32_t kind, uint32_t m,
+                             uint32_t* counts);
 
 /* Segmented partition: segment s is [seg_off[s], seg_off[s+1]), each
  * partitioned independently in place of its input range.  seg_counts is

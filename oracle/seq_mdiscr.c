@@ -84,24 +84,50 @@ int oracle_seq_mdiscr_soa(uint64_t n, uint32_t ncols, const void* const* cols,
                           void* const* out_cols, uint32_t kind, uint32_t m,
                           uint32_t* counts) {
     if (!discr_ok(kind, m) || ncols == 0 || key_col >= ncols || !cols ||
-        !col_bytes || !out_cols || col_bytes[key_col] != 4)
+        !col_bytes || !out_cols || (col_bytes[key_col] != 4 && col_bytes[key_col] != 1))
         return -1;
     for (uint32_t j = 0; j < ncols; j++) {
         uint32_t b = col_bytes[j];
         if (b != 1 && b != 2 && b != 4 && b != 8) return -1;
         if (n && (!cols[j] || !out_cols[j])) return -1;
     }
-    const uint32_t* key = (const uint32_t*)cols[key_col];
+    const void* key = cols[key_col];
+    const int key8 = col_bytes[key_col] == 1;
+#define KEY(i) (key8 ? (uint32_t)((const uint8_t*)key)[i] : ((const uint32_t*)key)[i])
     uint64_t* red = (uint64_t*)calloc(m, sizeof(uint64_t));
     uint64_t* cur = (uint64_t*)calloc(m, sizeof(uint64_t));
     if (!red || !cur) { free(red); free(cur); return -2; }
-    for (uint64_t i = 0; i < n; i++) red[discr(key[i], kind, m)]++;
+    for (uint64_t i = 0; i < n; i++) red[discr(KEY(i), kind, m)]++;
     uint64_t acc = 0;
     for (uint32_t k = 0; k < m; k++) { cur[k] = acc; acc += red[k]; }
     for (uint64_t i = 0; i < n; i++) {
-        uint64_t d = cur[discr(key[i], kind, m)]++;
+        uint64_t d = cur[discr(KEY(i), kind, m)]++;
         for (uint32_t j = 0; j < ncols; j++)
             copy_elem(out_cols[j], d, cols[j], i, col_bytes[j]);
+    }
+    if (counts)
+        for (uint32_t k = 0; k < m; k++) counts[k] = (uint32_t)red[k];
+    free(red); free(cur);
+#undef KEY
+    return 0;
+}
+
+/* MdiscrHost_arrays.cu.h:62-86 (seqMdiscr<DISCR,LEN>): element i is the len
+ * words in[i*len .. i*len+len), its class is D(word 0) (ModArray,
+ * :27-36), and the whole element is copied to its stable position. */
+int oracle_seq_mdiscr_arrays(uint64_t n, uint32_t len, const uint32_t* in,
+                             uint32_t* out, uint32_t kind, uint32_t m,
+                             uint32_t* counts) {
+    if (!discr_ok(kind, m) || len == 0 || (n && (!in || !out))) return -1;
+    uint64_t* red = (uint64_t*)calloc(m, sizeof(uint64_t));
+    uint64_t* cur = (uint64_t*)calloc(m, sizeof(uint64_t));
+    if (!red || !cur) { free(red); free(cur); return -2; }
+    for (uint64_t i = 0; i < n; i++) red[discr(in[i * len], kind, m)]++;
+    uint64_t acc = 0;
+    for (uint32_t k = 0; k < m; k++) { cur[k] = acc; acc += red[k]; }
+    for (uint64_t i = 0; i < n; i++) {
+        uint64_t d = cur[discr(in[i * len], kind, m)]++;
+        memcpy(out + d * len, in + i * len, len * sizeof(uint32_t));
     }
     if (counts)
         for (uint32_t k = 0; k < m; k++) counts[k] = (uint32_t)red[k];

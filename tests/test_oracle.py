@@ -216,3 +216,33 @@ def test_divmagic_exhaustive_divisors(tmp_path):
     subprocess.run(["gcc", "-O2", "-std=c11", "-I", inc, "-o", exe, src], check=True)
     r = subprocess.run([exe], capture_output=True, text=True)
     assert r.returncode == 0, r.stdout + r.stderr
+
+
+@pytest.mark.parametrize("length", [1, 2, 3, 8])
+def test_arrays_oracle_closed_form(length):
+    """Array elements (MdiscrHost_arrays.cu.h:62-86): rows of `length` words,
+    class from word 0, whole rows in stable class order."""
+    n, m = 3001, 7
+    x = W.uniform_u31(n * length, length)
+    rows = x.reshape(n, length)
+    out, cnt = oracle.seq_mdiscr_arrays(x, length, m)
+    perm = np.argsort(rows[:, 0] % m, kind="stable")
+    assert np.array_equal(out.reshape(n, length), rows[perm])
+    assert np.array_equal(cnt, np.bincount(rows[:, 0] % m, minlength=m))
+    # the same partition through the struct-of-arrays oracle, column by column
+    cols = [np.ascontiguousarray(rows[:, j]) for j in range(length)]
+    outs, cnt2 = oracle.seq_mdiscr_soa(cols, m)
+    assert np.array_equal(np.stack(outs, axis=1), out.reshape(n, length))
+    assert np.array_equal(cnt, cnt2)
+
+
+def test_soa_oracle_one_byte_class_key():
+    """A 1-byte key column is a class array: class = value (values < m)."""
+    n, m = 20000, 13
+    cls = (W.uniform_u31(n, 5) % m).astype(np.uint8)
+    pay = W.uniform_u31(n, 6)
+    outs, cnt = oracle.seq_mdiscr_soa([pay, cls], m, key_col=1)
+    perm = np.argsort(cls, kind="stable")
+    assert np.array_equal(outs[0], pay[perm])
+    assert np.array_equal(outs[1], cls[perm])
+    assert np.array_equal(cnt, np.bincount(cls, minlength=m))

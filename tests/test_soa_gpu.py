@@ -130,9 +130,38 @@ def test_soa_invalid(ctx):
     import torch
     import mdiscriminator_amd as md
     a = torch.zeros(100, dtype=torch.int32, device="cuda:0")
-    b8 = torch.zeros(100, dtype=torch.uint8, device="cuda:0")
+    b16 = torch.zeros(100, dtype=torch.int16, device="cuda:0")
     o = torch.zeros(100, dtype=torch.int32, device="cuda:0")
     with pytest.raises(md.MdiscrError):
-        ctx.partition_soa([b8, a], [torch.empty_like(b8), o], md.Mod4, key_col=0)  # 1-byte key
+        ctx.partition_soa([b16, a], [torch.empty_like(b16), o], md.Mod4, key_col=0)  # 2-byte key
     with pytest.raises(md.MdiscrError):
         ctx.partition_soa([a], [a], md.Mod4)                                        # aliasing
+
+
+@pytest.mark.parametrize("length", [1, 2, 3, 4, 5, 8])
+def test_array_elements(ctx, length):
+    """Array-valued elements (mdiscr<DISCR,LEN>, MdiscrHost_arrays.cu.h:250-470)."""
+    import torch
+    import mdiscriminator_amd as md
+    for n, m in ((1, 4), (8191, 8), (100003, 5), (1 << 18, 64)):
+        x = W.uniform_u31(n * length, 900 + length + n)
+        d_in = torch.from_numpy(x.view(np.int32)).to("cuda:0")
+        d_out = torch.full_like(d_in, -1)
+        d_cnt = torch.empty(m, dtype=torch.int32, device="cuda:0")
+        print(f"arrays len={length} n={n} m={m}", flush=True)
+        ctx.partition_arrays(d_in, d_out, length, md.Mod(m), d_counts=d_cnt)
+        torch.cuda.synchronize()
+        ref, ref_cnt = oracle.seq_mdiscr_arrays(x, length, m)
+        assert first_diff(d_out.cpu().numpy().view(np.uint32), ref) < 0
+        assert np.array_equal(d_cnt.cpu().numpy().view(np.uint32), ref_cnt)
+
+
+def test_class_array_key(ctx):
+    """A 1-byte class column as the discriminator, with 4- and 8-byte payloads."""
+    import mdiscriminator_amd as md
+    for m, n in ((3, 50001), (16, 1 << 20), (64, 300000)):
+        cls = (W.uniform_u31(n, m) % m).astype(np.uint8)
+        pay = W.uniform_u31(n, m + 1)
+        wide = W.splitmix64(n, m + 2)
+        check_soa(ctx, [pay, cls], md.Mod(m), key_col=1, label=f"class key m={m}")
+        check_soa(ctx, [cls, wide, pay], md.Mod(m), key_col=0, label=f"class key wide m={m}")
