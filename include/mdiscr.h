@@ -121,6 +121,58 @@ mdiscr_status mdiscr_partition_arrays(mdiscr_ctx* ctx, const uint32_t* d_in,
                                       mdiscr_discr d, uint32_t* d_counts,
                                       void* stream);
 
+/* ---- general discriminators ------------------------------------------ */
+
+/* Two-way filter predicates on unsigned 32-bit keys. */
+typedef enum {
+    MDISCR_PRED_LT = 0,     /* x <  a */
+    MDISCR_PRED_LE = 1,     /* x <= a */
+    MDISCR_PRED_GT = 2,     /* x >  a */
+    MDISCR_PRED_GE = 3,     /* x >= a */
+    MDISCR_PRED_EQ = 4,     /* x == a */
+    MDISCR_PRED_NE = 5,     /* x != a */
+    MDISCR_PRED_BITS = 6,   /* (x & a) == b */
+    MDISCR_PRED_RANGE = 7   /* a <= x < b */
+} mdiscr_pred_op;
+
+typedef struct {
+    uint32_t op;  /* mdiscr_pred_op */
+    uint32_t a;
+    uint32_t b;   /* BITS and RANGE only */
+} mdiscr_pred;
+
+/* parFilter cond arr (ParFilter.hs:8-21; no CUDA counterpart): d_out holds
+ * the elements of d_in where the predicate holds, in input order, then the
+ * rest, in input order.  d_counts (device, 2 entries, may be NULL) receives
+ * [kept, rest]; the reference's flag array (write [0,i] [i,n-i]) is
+ * mdiscr_sizes_from_counts(ctx, d_counts, 2, n, ...). */
+mdiscr_status mdiscr_filter_u32(mdiscr_ctx* ctx, const uint32_t* d_in,
+                                uint32_t* d_out, uint64_t n, mdiscr_pred pred,
+                                uint32_t* d_counts, void* stream);
+
+/* The filter over struct-of-arrays records: the predicate reads the 4-byte
+ * key column and every column moves, as in mdiscr_partition_soa. */
+mdiscr_status mdiscr_filter_soa(mdiscr_ctx* ctx, uint32_t ncols,
+                                const void* const* d_cols,
+                                const uint32_t* col_bytes, uint32_t key_col,
+                                void* const* d_out_cols, uint64_t n,
+                                mdiscr_pred pred, uint32_t* d_counts,
+                                void* stream);
+
+/* Multi-column discriminator: the alternative body of ModTuple4/ModTuple<N>
+ * ::apply (MdiscrHost_tuples.cu.h:27,46, z ? x % N : y % N):
+ * class(i) = cols[sel_col][i] != 0 ? D(cols[a_col][i]) : D(cols[b_col][i]).
+ * The three columns hold 1- or 4-byte values and may coincide; every column
+ * moves, as in mdiscr_partition_soa. */
+mdiscr_status mdiscr_partition_soa_select(mdiscr_ctx* ctx, uint32_t ncols,
+                                          const void* const* d_cols,
+                                          const uint32_t* col_bytes,
+                                          uint32_t sel_col, uint32_t a_col,
+                                          uint32_t b_col,
+                                          void* const* d_out_cols, uint64_t n,
+                                          mdiscr_discr d, uint32_t* d_counts,
+                                          void* stream);
+
 /* ---- segmented partition (MdiscrSegmHost.cu.h:36-230) ---------------- */
 
 /* Each segment s = [d_seg_off[s], d_seg_off[s+1]) of d_in is partitioned

@@ -15,6 +15,10 @@ mirrors the reference's host interface (asbjornvg/mdiscriminator, cuda_code/):
         new_sizes) (MdiscrSegmHost.cu.h:36-42)              segment_sizes, out, new_sizes)
     tupleToSizeArray(counts, sizes, n)                  sizes_from_counts(...)
         (HelpersCommon.h:295-311)
+    parFilter cond arr -> (arr', flags)                 par_filter(Pred, d_in, d_out,
+        (haskell_code/ParFilter.hs:8-21)                    d_flags) -> kept
+    ModTuple<N>: z ? x % N : y % N                      Context.partition_soa_select
+        (MdiscrHost_tuples.cu.h:27,46)
 
 Device buffers are torch tensors on the GPU (plumbing only: all compute runs in
 the HIP kernels of libmdiscr.so).  Errors raise MdiscrError with the library's
@@ -28,7 +32,8 @@ from ._lib import MdiscrError, DISCR_MOD, DISCR_MASK, MAX_CLASSES, MAX_ELEMS
 
 __all__ = [
     "Discr", "Mod", "Mask", "Mod4", "Mod8", "Context", "Server", "MdiscrError", "default_context",
-    "mdiscr", "mdiscr_tuples", "mdiscr_segm", "sizes_from_counts", "MAX_CLASSES",
+    "mdiscr", "mdiscr_tuples", "mdiscr_segm", "sizes_from_counts", "MAX_CLASSES", "Pred",
+    "par_filter",
 ]
 
 
@@ -58,6 +63,52 @@ def Mask(n):
 
 Mod4 = Mask(4)  # HelpersCommon.h:587-596
 Mod8 = Mask(8)  # HelpersCommon.h:840-849
+
+
+@dataclass(frozen=True)
+class Pred:
+    """A two-way filter predicate on unsigned 32-bit keys; elements where it
+    holds come first (class 0), the rest after (class 1)."""
+    op: int
+    a: int = 0
+    b: int = 0
+
+    @staticmethod
+    def lt(a):
+        return Pred(_lib.PRED_LT, a)
+
+    @staticmethod
+    def le(a):
+        return Pred(_lib.PRED_LE, a)
+
+    @staticmethod
+    def gt(a):
+        return Pred(_lib.PRED_GT, a)
+
+    @staticmethod
+    def ge(a):
+        return Pred(_lib.PRED_GE, a)
+
+    @staticmethod
+    def eq(a):
+        return Pred(_lib.PRED_EQ, a)
+
+    @staticmethod
+    def ne(a):
+        return Pred(_lib.PRED_NE, a)
+
+    @staticmethod
+    def bits(mask, value):
+        """(x & mask) == value"""
+        return Pred(_lib.PRED_BITS, mask, value)
+
+    @staticmethod
+    def range(lo, hi):
+        """lo <= x < hi"""
+        return Pred(_lib.PRED_RANGE, lo, hi)
+
+    def c(self):
+        return _lib.MdiscrPred(self.op, self.a & 0xFFFFFFFF, self.b & 0xFFFFFFFF)
 
 
 def _ptr(t):
@@ -134,6 +185,39 @@ class Context:
         st = self.lib.mdiscr_partition_soa(self.h, k, pin, nb, key_col, pout, n, discr.c(),
                                            _ptr(d_counts), _stream(stream))
         self._check(st, "mdiscr_partition_soa")
+
+    def partition_soa_select(self, cols, out_cols, discr, sel_col, a_col, b_col, d_counts=None,
+                             n=None, stream=None):
+        """class = cols[sel] != 0 ? D(cols[a]) : D(cols[b]) (the alternative
+        ModTuple body, MdiscrHost_tuples.cu.h:27,46); 1- or 4-byte columns."""
+        k = len(cols)
+        n = cols[0].numel() if n is None else n
+        pin = (ctypes.c_void_p * k)(*[_ptr(c) for c in cols])
+        pout = (ctypes.c_void_p * k)(*[_ptr(c) for c in out_cols])
+        nb = (ctypes.c_uint32 * k)(*[c.element_size() for c in cols])
+        st = self.lib.mdiscr_partition_soa_select(self.h, k, pin, nb, sel_col, a_col, b_col, pout,
+                                                  n, discr.c(), _ptr(d_counts), _stream(stream))
+        self._check(st, "mdiscr_partition_soa_select")
+
+    # -- two-way filter ----------------------------------------------------------
+    def filter_u32(self, d_in, d_out, pred, d_counts=None, n=None, stream=None):
+        """parFilter (ParFilter.hs:8-21): kept elements, then the rest, both in
+        input order; d_counts (2 entries) = [kept, rest]."""
+        n = d_in.numel() if n is None else n
+        st = self.lib.mdiscr_filter_u32(self.h, _ptr(d_in), _ptr(d_out), n, pred.c(),
+                                        _ptr(d_counts), _stream(stream))
+        self._check(st, "mdiscr_filter_u32")
+
+    def filter_soa(self, cols, out_cols, pred, key_col=0, d_counts=None, n=None, stream=None):
+        """The filter over records: predicate on the 4-byte key column."""
+        k = len(cols)
+        n = cols[key_col].numel() if n is None else n
+        pin = (ctypes.c_void_p * k)(*[_ptr(c) for c in cols])
+        pout = (ctypes.c_void_p * k)(*[_ptr(c) for c in out_cols])
+        nb = (ctypes.c_uint32 * k)(*[c.element_size() for c in cols])
+        st = self.lib.mdiscr_filter_soa(self.h, k, pin, nb, key_col, pout, n, pred.c(),
+                                        _ptr(d_counts), _stream(stream))
+        self._check(st, "mdiscr_filter_soa")
 
     # -- array-valued elements --------------------------------------------------
     def partition_arrays(self, d_in, d_out, length, discr, d_counts=None, n=None, stream=None):
@@ -311,6 +395,21 @@ def mdiscr_segm(discr, num_elems, in_array, segment_sizes, out_array, new_sizes)
     (MdiscrSegmHost.cu.h:36-42)."""
     default_context(in_array.device.index).segm_flags(discr, num_elems, in_array, segment_sizes,
                                                       out_array, new_sizes)
+
+
+def par_filter(pred, d_in, d_out, d_flags=None):
+    """parFilter cond arr -> (arr', flags) (haskell_code/ParFilter.hs:8-21).
+    Writes the filtered array to d_out and, when given, the reference's
+    n-length flag array (write [0,i] [i,n-i]) to d_flags; returns i, the
+    number of elements kept."""
+    import torch
+    ctx = default_context(d_in.device.index)
+    n = d_in.numel()
+    counts = torch.empty(2, dtype=torch.int32, device=d_in.device)
+    ctx.filter_u32(d_in, d_out, pred, d_counts=counts)
+    if d_flags is not None and n:
+        ctx.sizes_from_counts(counts, 2, n, d_flags)
+    return int(counts[0].item())
 
 
 def sizes_from_counts(d_counts, m, n, d_sizes):

@@ -11,12 +11,17 @@ LIB_PATH = os.path.join(_PKG, "libmdiscr.so")
 
 OK, EINVAL, ENOMEM, EDEVICE, EBUSY, EUNSUPPORTED = range(6)
 DISCR_MOD, DISCR_MASK = 0, 1
+PRED_LT, PRED_LE, PRED_GT, PRED_GE, PRED_EQ, PRED_NE, PRED_BITS, PRED_RANGE = range(8)
 MAX_CLASSES = 64
 MAX_ELEMS = 0xFFFF0000
 
 
 class MdiscrDiscr(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_uint32), ("m", ctypes.c_uint32)]
+
+
+class MdiscrPred(ctypes.Structure):
+    _fields_ = [("op", ctypes.c_uint32), ("a", ctypes.c_uint32), ("b", ctypes.c_uint32)]
 
 
 class MdiscrRequest(ctypes.Structure):
@@ -61,6 +66,10 @@ SIGNATURES = {
     "mdiscr_u32": (_int, [_vp, _u64, _u32, _vp, _vp, MdiscrDiscr, _vp]),
     "mdiscr_partition_soa": (_int, [_vp, _u32, _vp, _vp, _u32, _vp, _u64, MdiscrDiscr, _vp, _vp]),
     "mdiscr_partition_arrays": (_int, [_vp, _vp, _vp, _u64, _u32, MdiscrDiscr, _vp, _vp]),
+    "mdiscr_filter_u32": (_int, [_vp, _vp, _vp, _u64, MdiscrPred, _vp, _vp]),
+    "mdiscr_filter_soa": (_int, [_vp, _u32, _vp, _vp, _u32, _vp, _u64, MdiscrPred, _vp, _vp]),
+    "mdiscr_partition_soa_select": (_int, [_vp, _u32, _vp, _vp, _u32, _u32, _u32, _vp, _u64,
+                                           MdiscrDiscr, _vp, _vp]),
     "mdiscr_partition_segmented": (_int, [_vp, _vp, _vp, _u64, _vp, _vp, _u64, MdiscrDiscr, _vp,
                                           _vp]),
     "mdiscr_segm_flags": (_int, [_vp, _u64, _vp, _vp, _vp, _vp, MdiscrDiscr, _vp]),

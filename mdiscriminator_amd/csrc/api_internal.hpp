@@ -40,6 +40,12 @@ void timing_end(mdiscr_ctx* ctx, hipStream_t s, TimedLaunch& t, const char* name
 mdiscr_status check_discr(mdiscr_ctx* ctx, mdiscr_discr d);
 uint32_t class_bits(uint32_t m);
 
+// A two-way filter predicate normalised to one unsigned window test:
+// holds(x) = ((x & w[0]) - w[1] <= w[2]) != inv, with w[3] = inv ^ 1 so that
+// class(x) = window ^ w[3] (0 where the predicate holds).  False for an
+// unknown op.
+bool pred_window(mdiscr_pred p, uint32_t w[4]);
+
 inline bool ranges_overlap(const void* a, const void* b, uint64_t bytes) {
     const uintptr_t x = (uintptr_t)a, y = (uintptr_t)b;
     return x < y + bytes && y < x + bytes;

@@ -33,6 +33,23 @@ struct DiscrMod {
     }
 };
 
+// Two-way filter (parFilter, ParFilter.hs:8-21): class 0 where the predicate
+// holds, class 1 elsewhere.  Every predicate is normalised on the host to one
+// unsigned window test, holds = ((x & mask) - lo <= hi) != inv, so the device
+// spends an AND, a SUB and a compare per element whatever the predicate.
+struct DiscrPred {
+    uint32_t mask, lo, hi, flip;  // flip = inv ^ 1
+    __device__ __forceinline__ uint32_t operator()(uint32_t x) const {
+        return (uint32_t)((x & mask) - lo <= hi) ^ flip;
+    }
+};
+
+// The loaded value already is the class (class sources that apply their own
+// discriminator, e.g. a selection between two key columns).
+struct DiscrIdent {
+    __device__ __forceinline__ uint32_t operator()(uint32_t x) const { return x; }
+};
+
 // ---- wave64 primitives ------------------------------------------------------
 
 __device__ __forceinline__ uint32_t lane_id() {

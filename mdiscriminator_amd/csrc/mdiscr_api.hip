@@ -255,7 +255,72 @@ mdiscr_status dispatch_plain(mdiscr_ctx* ctx, const uint32_t* in, uint32_t* out,
     return ctx_fail(ctx, MDISCR_EUNSUPPORTED, "m=%u", m);
 }
 
+// Argument checks, the n = 0 case and the workspace of one plain-path call.
+// On success *ws is the workspace, or null when nothing is left to launch.
+mdiscr_status plain_prologue(mdiscr_ctx* ctx, const uint32_t* d_in, uint32_t* d_out, uint64_t n,
+                             uint32_t m, uint32_t* d_counts, hipStream_t s, uint32_t** ws) {
+    *ws = nullptr;
+    if (n > MDISCR_MAX_ELEMS) return ctx_fail(ctx, MDISCR_EINVAL, "n=%llu too large", (unsigned long long)n);
+    if (n && (!d_in || !d_out)) return ctx_fail(ctx, MDISCR_EINVAL, "null data pointer");
+    if (n && ranges_overlap(d_in, d_out, n * 4))
+        return ctx_fail(ctx, MDISCR_EINVAL, "output overlaps input");
+    mdiscr_status st = ctx_hip(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    if (st) return st;
+    if (n == 0) {
+        if (d_counts) {
+            hipLaunchKernelGGL(k_fill_u32<>, dim3(1), dim3(64), 0, s, d_counts, 0u, m);
+            return ctx_hip(ctx, hipGetLastError(), "fill launch");
+        }
+        return MDISCR_OK;
+    }
+    const uint64_t T = (n + kTile - 1) / kTile;
+    const uint64_t ld = (T + 3) & ~3ull;
+    void* w = nullptr;
+    st = ctx_workspace(ctx, (ld + 1) * m * sizeof(uint32_t), &w);
+    if (st) return st;
+    *ws = static_cast<uint32_t*>(w);
+    return MDISCR_OK;
+}
+
 }  // namespace
+
+bool pred_window(mdiscr_pred p, uint32_t w[4]) {
+    // {mask, lo, hi, inv}; "never" is the always-true window inverted.
+    uint32_t mask = ~0u, lo = 0, hi = 0, inv = 0;
+    switch (p.op) {
+        case MDISCR_PRED_LT:  // x < a  <=>  x <= a-1 (never for a = 0)
+        case MDISCR_PRED_GE:  // not (x < a)
+            if (p.a == 0) { mask = 0; hi = ~0u; inv = 1; }
+            else hi = p.a - 1;
+            if (p.op == MDISCR_PRED_GE) inv ^= 1u;
+            break;
+        case MDISCR_PRED_LE:
+        case MDISCR_PRED_GT:  // not (x <= a)
+            hi = p.a;
+            inv = p.op == MDISCR_PRED_GT;
+            break;
+        case MDISCR_PRED_EQ:
+        case MDISCR_PRED_NE:
+            lo = p.a;
+            inv = p.op == MDISCR_PRED_NE;
+            break;
+        case MDISCR_PRED_BITS:  // (x & a) - b == 0
+            mask = p.a;
+            lo = p.b;
+            break;
+        case MDISCR_PRED_RANGE:  // x - a <= b - a - 1 (never for b <= a)
+            if (p.b <= p.a) { mask = 0; hi = ~0u; inv = 1; }
+            else { lo = p.a; hi = p.b - p.a - 1; }
+            break;
+        default:
+            return false;
+    }
+    w[0] = mask;
+    w[1] = lo;
+    w[2] = hi;
+    w[3] = inv ^ 1u;
+    return true;
+}
 
 mdiscr_status check_discr(mdiscr_ctx* ctx, mdiscr_discr d) {
     if (d.m == 0 || d.m > MDISCR_MAX_CLASSES)
@@ -284,30 +349,28 @@ mdiscr_status mdiscr_partition_u32(mdiscr_ctx* ctx, const uint32_t* d_in, uint32
     if (!ctx) return MDISCR_EINVAL;
     mdiscr_status st = check_discr(ctx, d);
     if (st) return st;
-    if (n > MDISCR_MAX_ELEMS) return ctx_fail(ctx, MDISCR_EINVAL, "n=%llu too large", (unsigned long long)n);
-    if (n && (!d_in || !d_out)) return ctx_fail(ctx, MDISCR_EINVAL, "null data pointer");
-    if (n && ranges_overlap(d_in, d_out, n * 4))
-        return ctx_fail(ctx, MDISCR_EINVAL, "output overlaps input");
     hipStream_t s = (hipStream_t)stream;
-    st = ctx_hip(ctx, hipSetDevice(ctx->device), "hipSetDevice");
-    if (st) return st;
-    if (n == 0) {
-        if (d_counts) {
-            hipLaunchKernelGGL(k_fill_u32<>, dim3(1), dim3(64), 0, s, d_counts, 0u, d.m);
-            return ctx_hip(ctx, hipGetLastError(), "fill launch");
-        }
-        return MDISCR_OK;
-    }
-    const uint64_t T = (n + kTile - 1) / kTile;
-    const uint64_t ld = (T + 3) & ~3ull;
-    void* ws = nullptr;
-    st = ctx_workspace(ctx, (ld + 1) * d.m * sizeof(uint32_t), &ws);
-    if (st) return st;
+    uint32_t* ws = nullptr;
+    st = plain_prologue(ctx, d_in, d_out, n, d.m, d_counts, s, &ws);
+    if (st || !ws) return st;
     if (d.kind == MDISCR_DISCR_MASK || (d.m & (d.m - 1)) == 0)
         return dispatch_plain(ctx, d_in, d_out, (uint32_t)n, DiscrMask{d.m - 1}, d.m, d_counts, s,
-                              (uint32_t*)ws);
+                              ws);
     return dispatch_plain(ctx, d_in, d_out, (uint32_t)n, DiscrMod{mdiscr_divmagic_make(d.m)}, d.m,
-                          d_counts, s, (uint32_t*)ws);
+                          d_counts, s, ws);
+}
+
+mdiscr_status mdiscr_filter_u32(mdiscr_ctx* ctx, const uint32_t* d_in, uint32_t* d_out,
+                                uint64_t n, mdiscr_pred pred, uint32_t* d_counts, void* stream) {
+    if (!ctx) return MDISCR_EINVAL;
+    uint32_t w[4];
+    if (!pred_window(pred, w)) return ctx_fail(ctx, MDISCR_EINVAL, "unknown predicate %u", pred.op);
+    hipStream_t s = (hipStream_t)stream;
+    uint32_t* ws = nullptr;
+    mdiscr_status st = plain_prologue(ctx, d_in, d_out, n, 2, d_counts, s, &ws);
+    if (st || !ws) return st;
+    return launch_plain<1>(ctx, d_in, d_out, (uint32_t)n, DiscrPred{w[0], w[1], w[2], w[3]}, 2u,
+                           d_counts, s, ws);
 }
 
 mdiscr_status mdiscr_u32(mdiscr_ctx* ctx, uint64_t n, uint32_t num_hwd_thds,

@@ -5,6 +5,8 @@
 // columns of 1/2/4/8-byte elements with any column as the 32-bit key.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "api_internal.hpp"
 #include "soa_kernels.hpp"
 
@@ -31,8 +33,8 @@ uint32_t soa_tile(bool wide, uint32_t nbits) {
     return nbits >= 6 && !wide ? SoaGeom<uint32_t, 6>::TILE : SoaGeom<uint64_t, 0>::TILE;
 }
 
-template <int NBITS, typename KT, typename ST, class Discr>
-mdiscr_status launch_soa(mdiscr_ctx* ctx, const SoaCols& cols, uint32_t n, Discr dis,
+template <int NBITS, typename ST, class Src, class Discr>
+mdiscr_status launch_soa(mdiscr_ctx* ctx, const SoaCols& cols, Src src, uint32_t n, Discr dis,
                          uint32_t m, uint32_t* user_counts, hipStream_t s, uint32_t* ws) {
     constexpr int NW = SoaGeom<ST, NBITS>::NW, R = SoaGeom<ST, NBITS>::R;
     constexpr uint32_t TILE = SoaGeom<ST, NBITS>::TILE;
@@ -42,81 +44,121 @@ mdiscr_status launch_soa(mdiscr_ctx* ctx, const SoaCols& cols, uint32_t n, Discr
     const uint32_t Kp = (T + kCT - 1) / kCT;
     uint32_t* counts = ws;
     uint32_t* totals = ws + (size_t)m * ld;
-    const KT* key = static_cast<const KT*>(cols.in[cols.key]);
-    const uint32_t kstride = cols.stride[cols.key];
     TimedLaunch tl;
     timing_begin(ctx, s, tl);
-    if (sizeof(KT) == 4 && kstride == 1 && ((uintptr_t)key & 15u) == 0)
-        hipLaunchKernelGGL((k_chunk_count<NW, R, kCCT, NBITS, true, Discr>), dim3(Kc),
-                           dim3(NW * kWave), 0, s, reinterpret_cast<const uint32_t*>(key), n, dis,
-                           counts, ld, m);
-    else
-        hipLaunchKernelGGL((k_key_count<NW, R, kCCT, NBITS, KT, Discr>), dim3(Kc), dim3(NW * kWave),
-                           0, s, key, kstride, n, dis, counts, ld, m);
+    bool vec = false;
+    if constexpr (std::is_same_v<Src, KeySrc<uint32_t>>)
+        vec = src.stride == 1 && ((uintptr_t)src.key & 15u) == 0;
+    if (vec) {
+        if constexpr (std::is_same_v<Src, KeySrc<uint32_t>>)
+            hipLaunchKernelGGL((k_chunk_count<NW, R, kCCT, NBITS, true, Discr>), dim3(Kc),
+                               dim3(NW * kWave), 0, s, src.key, n, dis, counts, ld, m);
+    } else {
+        hipLaunchKernelGGL((k_src_count<NW, R, kCCT, NBITS, Src, Discr>), dim3(Kc),
+                           dim3(NW * kWave), 0, s, src, n, dis, counts, ld, m);
+    }
     timing_end(ctx, s, tl, "soa_count");
     timing_begin(ctx, s, tl);
     hipLaunchKernelGGL((k_tile_scan<1024, 64>), dim3(m), dim3(1024), 0, s, counts, T, ld, totals,
                        user_counts, (const uint32_t*)nullptr);
     timing_end(ctx, s, tl, "soa_scan");
     timing_begin(ctx, s, tl);
-    hipLaunchKernelGGL((k_partition_soa<NW, R, kCT, NBITS, KT, ST, Discr>), dim3(Kp),
-                       dim3(NW * kWave), 0, s, cols, n, dis, counts, totals, ld, m);
+    hipLaunchKernelGGL((k_partition_soa<NW, R, kCT, NBITS, Src, ST, Discr>), dim3(Kp),
+                       dim3(NW * kWave), 0, s, cols, src, n, dis, counts, totals, ld, m);
     timing_end(ctx, s, tl, "soa_partition");
     return ctx_hip(ctx, hipGetLastError(), "SoA partition launch");
 }
 
-template <typename KT, typename ST, class Discr>
-mdiscr_status dispatch_soa(mdiscr_ctx* ctx, const SoaCols& cols, uint32_t n, Discr dis,
-                           uint32_t m, uint32_t* user_counts, hipStream_t s, uint32_t* ws) {
+template <typename ST, class Src, class Discr>
+mdiscr_status dispatch_bits(mdiscr_ctx* ctx, const SoaCols& cols, Src src, uint32_t n, Discr dis,
+                            uint32_t m, uint32_t* user_counts, hipStream_t s, uint32_t* ws) {
     switch (class_bits(m)) {
-        case 0: return launch_soa<0, KT, ST>(ctx, cols, n, dis, m, user_counts, s, ws);
-        case 1: return launch_soa<1, KT, ST>(ctx, cols, n, dis, m, user_counts, s, ws);
-        case 2: return launch_soa<2, KT, ST>(ctx, cols, n, dis, m, user_counts, s, ws);
-        case 3: return launch_soa<3, KT, ST>(ctx, cols, n, dis, m, user_counts, s, ws);
-        case 4: return launch_soa<4, KT, ST>(ctx, cols, n, dis, m, user_counts, s, ws);
-        case 5: return launch_soa<5, KT, ST>(ctx, cols, n, dis, m, user_counts, s, ws);
-        case 6: return launch_soa<6, KT, ST>(ctx, cols, n, dis, m, user_counts, s, ws);
+        case 0: return launch_soa<0, ST>(ctx, cols, src, n, dis, m, user_counts, s, ws);
+        case 1: return launch_soa<1, ST>(ctx, cols, src, n, dis, m, user_counts, s, ws);
+        case 2: return launch_soa<2, ST>(ctx, cols, src, n, dis, m, user_counts, s, ws);
+        case 3: return launch_soa<3, ST>(ctx, cols, src, n, dis, m, user_counts, s, ws);
+        case 4: return launch_soa<4, ST>(ctx, cols, src, n, dis, m, user_counts, s, ws);
+        case 5: return launch_soa<5, ST>(ctx, cols, src, n, dis, m, user_counts, s, ws);
+        case 6: return launch_soa<6, ST>(ctx, cols, src, n, dis, m, user_counts, s, ws);
     }
     return ctx_fail(ctx, MDISCR_EUNSUPPORTED, "m=%u", m);
 }
 
-template <typename KT, typename ST>
-mdiscr_status dispatch_discr(mdiscr_ctx* ctx, const SoaCols& cols, uint32_t n, mdiscr_discr d,
-                             uint32_t* user_counts, hipStream_t s, uint32_t* ws) {
+template <typename ST, class Src>
+mdiscr_status dispatch_discr(mdiscr_ctx* ctx, const SoaCols& cols, Src src, uint32_t n,
+                             mdiscr_discr d, uint32_t* user_counts, hipStream_t s, uint32_t* ws) {
     if (d.kind == MDISCR_DISCR_MASK || (d.m & (d.m - 1)) == 0)
-        return dispatch_soa<KT, ST>(ctx, cols, n, DiscrMask{d.m - 1}, d.m, user_counts, s, ws);
-    return dispatch_soa<KT, ST>(ctx, cols, n, DiscrMod{mdiscr_divmagic_make(d.m)}, d.m,
-                                user_counts, s, ws);
+        return dispatch_bits<ST>(ctx, cols, src, n, DiscrMask{d.m - 1}, d.m, user_counts, s, ws);
+    return dispatch_bits<ST>(ctx, cols, src, n, DiscrMod{mdiscr_divmagic_make(d.m)}, d.m,
+                             user_counts, s, ws);
 }
 
-// Validated columns -> launch (shared by the SoA and array entry points).
-mdiscr_status run_soa(mdiscr_ctx* ctx, SoaCols& cols, uint64_t n, mdiscr_discr d,
+// What feeds the discriminator of one SoA call.
+enum class ClassKind { Key, Pred, Select };
+struct ClassSpec {
+    ClassKind kind = ClassKind::Key;
+    mdiscr_discr d{};                // Key, Select
+    uint32_t pw[4] = {0, 0, 0, 0};   // Pred: the normalised window (m = 2)
+    uint32_t sel = 0, a = 0, b = 0;  // Select: column indices
+    uint32_t m() const { return kind == ClassKind::Pred ? 2u : d.m; }
+};
+
+template <typename ST>
+mdiscr_status run_staged(mdiscr_ctx* ctx, const SoaCols& cols, uint32_t n, const ClassSpec& cs,
+                         uint32_t* d_counts, hipStream_t s, uint32_t* w) {
+    switch (cs.kind) {
+        case ClassKind::Key:
+            if (cols.bytes[cols.key] == 1)
+                return dispatch_discr<ST>(
+                    ctx, cols, KeySrc<uint8_t>{static_cast<const uint8_t*>(cols.in[cols.key]),
+                                               cols.stride[cols.key]},
+                    n, cs.d, d_counts, s, w);
+            return dispatch_discr<ST>(
+                ctx, cols, KeySrc<uint32_t>{static_cast<const uint32_t*>(cols.in[cols.key]),
+                                            cols.stride[cols.key]},
+                n, cs.d, d_counts, s, w);
+        case ClassKind::Pred:
+            return launch_soa<1, ST>(
+                ctx, cols,
+                KeySrc<uint32_t>{static_cast<const uint32_t*>(cols.in[cols.key]),
+                                 cols.stride[cols.key]},
+                n, DiscrPred{cs.pw[0], cs.pw[1], cs.pw[2], cs.pw[3]}, 2u, d_counts, s, w);
+        case ClassKind::Select:
+            return dispatch_discr<ST>(
+                ctx, cols,
+                SelectSrc{cols.in[cs.sel], cols.in[cs.a], cols.in[cs.b], cols.bytes[cs.sel] == 4u,
+                          cols.bytes[cs.a] == 4u, cols.bytes[cs.b] == 4u},
+                n, cs.d, d_counts, s, w);
+    }
+    return ctx_fail(ctx, MDISCR_EINVAL, "class kind");
+}
+
+// Validated columns -> launch (shared by the SoA, array, filter and select
+// entry points).
+mdiscr_status run_soa(mdiscr_ctx* ctx, SoaCols& cols, uint64_t n, const ClassSpec& cs,
                       uint32_t* d_counts, void* stream) {
     hipStream_t s = (hipStream_t)stream;
+    const uint32_t m = cs.m();
     mdiscr_status st = ctx_hip(ctx, hipSetDevice(ctx->device), "hipSetDevice");
     if (st) return st;
     if (n == 0) {
         if (d_counts) {
-            hipLaunchKernelGGL(k_fill_u32<>, dim3(1), dim3(64), 0, s, d_counts, 0u, d.m);
+            hipLaunchKernelGGL(k_fill_u32<>, dim3(1), dim3(64), 0, s, d_counts, 0u, m);
             return ctx_hip(ctx, hipGetLastError(), "fill launch");
         }
         return MDISCR_OK;
     }
     bool wide = false;
     for (uint32_t j = 0; j < cols.ncols; ++j) wide |= cols.bytes[j] > 4;
-    const uint64_t tile = soa_tile(wide, class_bits(d.m));
+    const uint64_t tile = soa_tile(wide, class_bits(m));
     const uint64_t T = (n + tile - 1) / tile;
     const uint64_t ld = (T + 3) & ~3ull;
     void* ws = nullptr;
-    st = ctx_workspace(ctx, (ld + 1) * d.m * sizeof(uint32_t), &ws);
+    st = ctx_workspace(ctx, (ld + 1) * m * sizeof(uint32_t), &ws);
     if (st) return st;
     uint32_t* w = static_cast<uint32_t*>(ws);
-    const bool key8 = cols.bytes[cols.key] == 1;
-    if (wide)
-        return key8 ? dispatch_discr<uint8_t, uint64_t>(ctx, cols, (uint32_t)n, d, d_counts, s, w)
-                    : dispatch_discr<uint32_t, uint64_t>(ctx, cols, (uint32_t)n, d, d_counts, s, w);
-    return key8 ? dispatch_discr<uint8_t, uint32_t>(ctx, cols, (uint32_t)n, d, d_counts, s, w)
-                : dispatch_discr<uint32_t, uint32_t>(ctx, cols, (uint32_t)n, d, d_counts, s, w);
+    return wide ? run_staged<uint64_t>(ctx, cols, (uint32_t)n, cs, d_counts, s, w)
+                : run_staged<uint32_t>(ctx, cols, (uint32_t)n, cs, d_counts, s, w);
 }
 
 // Input and output extents must not overlap, column against column.
@@ -137,23 +179,19 @@ mdiscr_status check_overlaps(mdiscr_ctx* ctx, const SoaCols& cols, uint64_t n) {
 
 }  // namespace
 
-extern "C" mdiscr_status mdiscr_partition_soa(mdiscr_ctx* ctx, uint32_t ncols,
-                                              const void* const* d_cols,
-                                              const uint32_t* col_bytes, uint32_t key_col,
-                                              void* const* d_out_cols, uint64_t n,
-                                              mdiscr_discr d, uint32_t* d_counts,
-                                              void* stream) {
-    if (!ctx) return MDISCR_EINVAL;
-    mdiscr_status st = check_discr(ctx, d);
-    if (st) return st;
+namespace {
+
+// Contiguous caller columns -> SoaCols, with the checks every SoA entry point
+// shares (count, element sizes, null and misaligned pointers, overlaps).
+mdiscr_status build_cols(mdiscr_ctx* ctx, uint32_t ncols, const void* const* d_cols,
+                         const uint32_t* col_bytes, void* const* d_out_cols, uint64_t n,
+                         SoaCols& cols) {
     if (ncols == 0 || ncols > (uint32_t)kMaxCols || !d_cols || !col_bytes || !d_out_cols)
         return ctx_fail(ctx, MDISCR_EINVAL, "need 1..%d columns", kMaxCols);
-    if (key_col >= ncols || (col_bytes[key_col] != 4 && col_bytes[key_col] != 1))
-        return ctx_fail(ctx, MDISCR_EINVAL, "key column must hold 4-byte keys or 1-byte classes");
     if (n > MDISCR_MAX_ELEMS) return ctx_fail(ctx, MDISCR_EINVAL, "n too large");
-    SoaCols cols = {};
+    cols = SoaCols{};
     cols.ncols = ncols;
-    cols.key = key_col;
+    cols.key = (uint32_t)kMaxCols;  // no column is written from registers
     for (uint32_t j = 0; j < ncols; ++j) {
         const uint32_t b = col_bytes[j];
         if (b != 1 && b != 2 && b != 4 && b != 8)
@@ -167,9 +205,73 @@ extern "C" mdiscr_status mdiscr_partition_soa(mdiscr_ctx* ctx, uint32_t ncols,
         cols.bytes[j] = b;
         cols.stride[j] = 1;
     }
-    st = check_overlaps(ctx, cols, n);
+    return check_overlaps(ctx, cols, n);
+}
+
+}  // namespace
+
+extern "C" mdiscr_status mdiscr_partition_soa(mdiscr_ctx* ctx, uint32_t ncols,
+                                              const void* const* d_cols,
+                                              const uint32_t* col_bytes, uint32_t key_col,
+                                              void* const* d_out_cols, uint64_t n,
+                                              mdiscr_discr d, uint32_t* d_counts,
+                                              void* stream) {
+    if (!ctx) return MDISCR_EINVAL;
+    mdiscr_status st = check_discr(ctx, d);
     if (st) return st;
-    return run_soa(ctx, cols, n, d, d_counts, stream);
+    SoaCols cols;
+    st = build_cols(ctx, ncols, d_cols, col_bytes, d_out_cols, n, cols);
+    if (st) return st;
+    if (key_col >= ncols || (cols.bytes[key_col] != 4 && cols.bytes[key_col] != 1))
+        return ctx_fail(ctx, MDISCR_EINVAL, "key column must hold 4-byte keys or 1-byte classes");
+    cols.key = key_col;
+    ClassSpec cs;
+    cs.d = d;
+    return run_soa(ctx, cols, n, cs, d_counts, stream);
+}
+
+extern "C" mdiscr_status mdiscr_filter_soa(mdiscr_ctx* ctx, uint32_t ncols,
+                                           const void* const* d_cols, const uint32_t* col_bytes,
+                                           uint32_t key_col, void* const* d_out_cols, uint64_t n,
+                                           mdiscr_pred pred, uint32_t* d_counts, void* stream) {
+    if (!ctx) return MDISCR_EINVAL;
+    ClassSpec cs;
+    cs.kind = ClassKind::Pred;
+    if (!pred_window(pred, cs.pw)) return ctx_fail(ctx, MDISCR_EINVAL, "unknown predicate %u", pred.op);
+    SoaCols cols;
+    mdiscr_status st = build_cols(ctx, ncols, d_cols, col_bytes, d_out_cols, n, cols);
+    if (st) return st;
+    if (key_col >= ncols || cols.bytes[key_col] != 4)
+        return ctx_fail(ctx, MDISCR_EINVAL, "the predicate reads a 4-byte key column");
+    cols.key = key_col;
+    return run_soa(ctx, cols, n, cs, d_counts, stream);
+}
+
+extern "C" mdiscr_status mdiscr_partition_soa_select(mdiscr_ctx* ctx, uint32_t ncols,
+                                                     const void* const* d_cols,
+                                                     const uint32_t* col_bytes, uint32_t sel_col,
+                                                     uint32_t a_col, uint32_t b_col,
+                                                     void* const* d_out_cols, uint64_t n,
+                                                     mdiscr_discr d, uint32_t* d_counts,
+                                                     void* stream) {
+    if (!ctx) return MDISCR_EINVAL;
+    mdiscr_status st = check_discr(ctx, d);
+    if (st) return st;
+    SoaCols cols;
+    st = build_cols(ctx, ncols, d_cols, col_bytes, d_out_cols, n, cols);
+    if (st) return st;
+    const uint32_t which[3] = {sel_col, a_col, b_col};
+    for (uint32_t c : which)
+        if (c >= ncols || (cols.bytes[c] != 1 && cols.bytes[c] != 4))
+            return ctx_fail(ctx, MDISCR_EINVAL,
+                            "select column %u must exist and hold 1- or 4-byte values", c);
+    ClassSpec cs;
+    cs.kind = ClassKind::Select;
+    cs.d = d;
+    cs.sel = sel_col;
+    cs.a = a_col;
+    cs.b = b_col;
+    return run_soa(ctx, cols, n, cs, d_counts, stream);
 }
 
 extern "C" mdiscr_status mdiscr_partition_arrays(mdiscr_ctx* ctx, const uint32_t* d_in,
@@ -194,5 +296,7 @@ extern "C" mdiscr_status mdiscr_partition_arrays(mdiscr_ctx* ctx, const uint32_t
         cols.bytes[j] = 4;
         cols.stride[j] = len;
     }
-    return run_soa(ctx, cols, n, d, d_counts, stream);
+    ClassSpec cs;
+    cs.d = d;
+    return run_soa(ctx, cols, n, cs, d_counts, stream);
 }

@@ -7,7 +7,9 @@
 // stride, so fixed-length array elements stored flat (the reference's array
 // variant, MdiscrHost_arrays.cu.h:14-96: class from element word 0, the whole
 // element moved) are LEN strided 32-bit columns with word 0 as the key.  A
-// 1-byte key column lets a caller-computed class array be the discriminator.
+// 1-byte key column lets a caller-computed class array be the discriminator;
+// a select source reads the class from one of two columns, chosen per element
+// by a third (the tuple discriminator's z ? x % N : y % N form).
 //
 // Same three launches as the plain path; the count pass reads only the key
 // column.  The partition ranks the key exactly as the plain kernel does,
@@ -39,11 +41,49 @@ __device__ __forceinline__ T ld_nt(const T* p) {
 }
 
 // ---------------------------------------------------------------------------
-// count pass over a (possibly strided, possibly 1-byte) key column.
-template <int NW, int R, int CT, int NBITS, typename KT, class Discr>
-__global__ __launch_bounds__(NW* kWave) void k_key_count(
-    const KT* __restrict__ key, uint32_t kstride, uint32_t n, Discr dis,
-    uint32_t* __restrict__ counts, uint32_t ld, uint32_t m) {
+// Class sources: the 32-bit value the discriminator sees for element idx.
+
+// One key column (4-byte keys or 1-byte classes) with an element stride.  The
+// values in registers are that column's elements, so the partition writes the
+// key column from registers.
+template <typename KT>
+struct KeySrc {
+    using KeyT = KT;
+    static constexpr bool kKeyInRegs = true;
+    const KT* key;
+    uint32_t stride;
+    __device__ __forceinline__ uint32_t operator()(uint32_t idx) const {
+        return (uint32_t)ld_nt(key + (size_t)idx * stride);
+    }
+};
+
+// z ? x : y over three contiguous columns of 1- or 4-byte values.  With one
+// discriminator D, z ? D(x) : D(y) = D(z ? x : y), so the select form of the
+// tuple discriminator costs one D per element.
+struct SelectSrc {
+    using KeyT = uint32_t;
+    static constexpr bool kKeyInRegs = false;
+    const void* sel;
+    const void* a;
+    const void* b;
+    uint32_t sel4, a4, b4;  // nonzero when that column holds 4-byte values
+    __device__ __forceinline__ static uint32_t word(const void* p, uint32_t w4, uint32_t idx) {
+        return w4 ? ld_nt(static_cast<const uint32_t*>(p) + idx)
+                  : (uint32_t)ld_nt(static_cast<const uint8_t*>(p) + idx);
+    }
+    __device__ __forceinline__ uint32_t operator()(uint32_t idx) const {
+        const uint32_t z = word(sel, sel4, idx);
+        const uint32_t x = word(a, a4, idx);
+        const uint32_t y = word(b, b4, idx);
+        return z ? x : y;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// count pass over a class source (a strided or 1-byte key column, or a select).
+template <int NW, int R, int CT, int NBITS, class Src, class Discr>
+__global__ __launch_bounds__(NW* kWave) void k_src_count(
+    Src src, uint32_t n, Discr dis, uint32_t* __restrict__ counts, uint32_t ld, uint32_t m) {
     constexpr int MC = 1 << NBITS;
     constexpr int TILE = NW * kWave * R;
     constexpr int THREADS = NW * kWave;
@@ -69,7 +109,7 @@ __global__ __launch_bounds__(NW* kWave) void k_key_count(
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             const uint32_t idx = tbase + k * THREADS + threadIdx.x;
-            v[k] = (full || idx < n) ? (uint32_t)ld_nt(key + (size_t)idx * kstride) : 0u;
+            v[k] = (full || idx < n) ? src(idx) : 0u;
         }
 #pragma unroll
         for (int k = 0; k < R; ++k) {
@@ -145,9 +185,9 @@ __device__ __forceinline__ void soa_write_keys(const uint32_t (&v)[R], void* __r
     __syncthreads();
 }
 
-template <int NW, int R, int CT, int NBITS, typename KT, typename ST, class Discr>
+template <int NW, int R, int CT, int NBITS, class Src, typename ST, class Discr>
 __global__ __launch_bounds__(NW* kWave) void k_partition_soa(
-    SoaCols cols, uint32_t n, Discr dis, const uint32_t* __restrict__ prefix,
+    SoaCols cols, Src src, uint32_t n, Discr dis, const uint32_t* __restrict__ prefix,
     const uint32_t* __restrict__ totals, uint32_t ld, uint32_t m) {
     constexpr int MC = 1 << NBITS;
     constexpr int TILE = NW * kWave * R;
@@ -157,8 +197,6 @@ __global__ __launch_bounds__(NW* kWave) void k_partition_soa(
     __shared__ uint32_t cursor[NW][MC];
     __shared__ uint32_t dpos[MC];
 
-    const KT* __restrict__ key = static_cast<const KT*>(cols.in[cols.key]);
-    const uint32_t kstride = cols.stride[cols.key];
     const uint32_t lane = lane_id();
     const uint32_t wave = threadIdx.x / kWave;
     const uint32_t chunk = xcd_tile(blockIdx.x, gridDim.x);
@@ -183,7 +221,7 @@ __global__ __launch_bounds__(NW* kWave) void k_partition_soa(
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const uint32_t idx = wbase + r * kWave + lane;
-            v[r] = (full || idx < n) ? (uint32_t)ld_nt(key + (size_t)idx * kstride) : 0u;
+            v[r] = (full || idx < n) ? src(idx) : 0u;
         }
         for (int c = lane; c < MC; c += kWave) cursor[wave][c] = 0;
         __builtin_amdgcn_wave_barrier();
@@ -218,9 +256,12 @@ __global__ __launch_bounds__(NW* kWave) void k_partition_soa(
 
         // 4. Every column through the staging buffer.
         for (uint32_t j = 0; j < cols.ncols; ++j) {
-            if (j == cols.key) {
-                soa_write_keys<KT, NW, R>(v, cols.out[j], cols.stride[j], slot, dest, valid, stage);
-                continue;
+            if constexpr (Src::kKeyInRegs) {
+                if (j == cols.key) {
+                    soa_write_keys<typename Src::KeyT, NW, R>(v, cols.out[j], cols.stride[j], slot,
+                                                              dest, valid, stage);
+                    continue;
+                }
             }
             const uint32_t ss = cols.stride[j];
             switch (cols.bytes[j]) {

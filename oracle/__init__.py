@@ -15,6 +15,8 @@ ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libseqref.so")
 
 MOD, MASK = 0, 1
+# two-way filter predicates (oracle.h ORACLE_PRED_*)
+PRED_LT, PRED_LE, PRED_GT, PRED_GE, PRED_EQ, PRED_NE, PRED_BITS, PRED_RANGE = range(8)
 
 _vp = ctypes.c_void_p
 _u32 = ctypes.c_uint32
@@ -36,6 +38,10 @@ def lib():
         L.oracle_seq_mdiscr_u32.argtypes = [_u64, _vp, _vp, _vp, _u32, _u32, _vp]
         L.oracle_seq_mdiscr_soa.argtypes = [_u64, _u32, _vp, _vp, _u32, _vp, _u32, _u32, _vp]
         L.oracle_seq_mdiscr_arrays.argtypes = [_u64, _u32, _vp, _vp, _u32, _u32, _vp]
+        L.oracle_par_filter_u32.argtypes = [_u64, _vp, _vp, _vp, _u32, _u32, _u32, _vp]
+        L.oracle_filter_soa.argtypes = [_u64, _u32, _vp, _vp, _u32, _vp, _u32, _u32, _u32, _vp]
+        L.oracle_seq_mdiscr_soa_select.argtypes = [_u64, _u32, _vp, _vp, _u32, _u32, _u32, _vp,
+                                                   _u32, _u32, _vp]
         L.oracle_seq_mdiscr_segm.argtypes = [_u64, _vp, _vp, _u64, _vp, _u32, _u32, _vp]
         L.oracle_segm_flags.argtypes = [_u64, _vp, _vp, _vp, _vp, _u32, _u32]
         L.oracle_sizes_from_counts.argtypes = [_vp, _u32, _u64, _vp]
@@ -88,6 +94,45 @@ def seq_mdiscr_soa(cols, m, key_col=0, kind=MOD):
     cnt = np.zeros(m, dtype=np.uint32)
     _check(lib().oracle_seq_mdiscr_soa(n, k, pin, nb, key_col, pout, kind, m, _p(cnt)),
            "oracle_seq_mdiscr_soa")
+    return outs, cnt
+
+
+def _col_arrays(cols):
+    cols = [np.ascontiguousarray(c) for c in cols]
+    outs = [np.empty_like(c) for c in cols]
+    k = len(cols)
+    pin = (_vp * k)(*[c.ctypes.data for c in cols])
+    pout = (_vp * k)(*[o.ctypes.data for o in outs])
+    nb = (_u32 * k)(*[c.dtype.itemsize for c in cols])
+    return cols, outs, pin, pout, nb
+
+
+def par_filter(x, op, a=0, b=0):
+    """parFilter (ParFilter.hs:8-21): (out, flags, [kept, rest])."""
+    x = np.ascontiguousarray(x, dtype=np.uint32)
+    out = np.empty_like(x)
+    flags = np.empty(len(x), dtype=np.uint32)
+    cnt = np.zeros(2, dtype=np.uint32)
+    _check(lib().oracle_par_filter_u32(len(x), _p(x), _p(out), _p(flags), op, a, b, _p(cnt)),
+           "oracle_par_filter_u32")
+    return out, flags, cnt
+
+
+def filter_soa(cols, op, a=0, b=0, key_col=0):
+    cols, outs, pin, pout, nb = _col_arrays(cols)
+    cnt = np.zeros(2, dtype=np.uint32)
+    _check(lib().oracle_filter_soa(len(cols[key_col]), len(cols), pin, nb, key_col, pout, op, a,
+                                   b, _p(cnt)), "oracle_filter_soa")
+    return outs, cnt
+
+
+def seq_mdiscr_soa_select(cols, m, sel_col, a_col, b_col, kind=MOD):
+    """class = cols[sel] != 0 ? D(cols[a]) : D(cols[b]) (MdiscrHost_tuples.cu.h:46)."""
+    cols, outs, pin, pout, nb = _col_arrays(cols)
+    cnt = np.zeros(m, dtype=np.uint32)
+    _check(lib().oracle_seq_mdiscr_soa_select(len(cols[0]), len(cols), pin, nb, sel_col, a_col,
+                                              b_col, pout, kind, m, _p(cnt)),
+           "oracle_seq_mdiscr_soa_select")
     return outs, cnt
 
 

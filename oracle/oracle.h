@@ -47,6 +47,40 @@ int oracle_seq_mdiscr_arrays(uint64_t n, uint32_t len, const uint32_t* in,
                              uint32_t* out, uint32_t kind, uint32_t m,
                              uint32_t* counts);
 
+/* Two-way filter predicates (unsigned 32-bit x). */
+enum {
+    ORACLE_PRED_LT = 0,    /* x <  a */
+    ORACLE_PRED_LE = 1,    /* x <= a */
+    ORACLE_PRED_GT = 2,    /* x >  a */
+    ORACLE_PRED_GE = 3,    /* x >= a */
+    ORACLE_PRED_EQ = 4,    /* x == a */
+    ORACLE_PRED_NE = 5,    /* x != a */
+    ORACLE_PRED_BITS = 6,  /* (x & a) == b */
+    ORACLE_PRED_RANGE = 7  /* a <= x < b */
+};
+
+/* parFilter (ParFilter.hs:8-21): the elements where the predicate holds, in
+ * order, then the rest, in order; flags is the reference's n-length flag
+ * array (write [0,i] [i,n-i]); counts = [i, n-i].  flags/counts may be NULL. */
+int oracle_par_filter_u32(uint64_t n, const uint32_t* in, uint32_t* out,
+                          uint32_t* flags, uint32_t op, uint32_t a, uint32_t b,
+                          uint32_t* counts);
+
+/* The filter over records: predicate on the 32-bit key column, every column
+ * moved; counts = [kept, rest]. */
+int oracle_filter_soa(uint64_t n, uint32_t ncols, const void* const* cols,
+                      const uint32_t* col_bytes, uint32_t key_col,
+                      void* const* out_cols, uint32_t op, uint32_t a, uint32_t b,
+                      uint32_t* counts);
+
+/* Multi-column discriminator z ? D(x) : D(y) (MdiscrHost_tuples.cu.h:27,46):
+ * class = cols[sel_col] != 0 ? D(cols[a_col]) : D(cols[b_col]), columns of
+ * 1- or 4-byte values. */
+int oracle_seq_mdiscr_soa_select(uint64_t n, uint32_t ncols, const void* const* cols,
+                                 const uint32_t* col_bytes, uint32_t sel_col,
+                                 uint32_t a_col, uint32_t b_col, void* const* out_cols,
+                                 uint32_t kind, uint32_t m, uint32_t* counts);
+
 /* Segmented partition: segment s is [seg_off[s], seg_off[s+1]), each
  * partitioned independently in place of its input range.  seg_counts is
  * nseg*m (may be NULL). */

@@ -135,6 +135,126 @@ int oracle_seq_mdiscr_arrays(uint64_t n, uint32_t len, const uint32_t* in,
     return 0;
 }
 
+/* Predicates of the two-way filter, each evaluated from its own definition
+ * (unsigned 32-bit comparisons).  -1 for an unknown op. */
+static int pred_eval(uint32_t op, uint32_t a, uint32_t b, uint32_t x) {
+    switch (op) {
+        case ORACLE_PRED_LT: return x < a;
+        case ORACLE_PRED_LE: return x <= a;
+        case ORACLE_PRED_GT: return x > a;
+        case ORACLE_PRED_GE: return x >= a;
+        case ORACLE_PRED_EQ: return x == a;
+        case ORACLE_PRED_NE: return x != a;
+        case ORACLE_PRED_BITS: return (x & a) == b;
+        case ORACLE_PRED_RANGE: return a <= x && x < b;
+    }
+    return -1;
+}
+
+/* parFilter cond arr (ParFilter.hs:8-21), step by step:
+ *   cs  = map cond arr, tfs = 1 where cond holds, ffs = 1 where it does not;
+ *   isT = scanInc (+) 0 tfs, i = last isT, isF = map (+ i) (scanInc (+) 0 ffs);
+ *   inds = [c ? iT-1 : iF-1], result = permute inds arr;
+ *   flags = write [0,i] [i,n-i] (replicate n 0): later writes win, and an
+ *   index that matches no position (i == n) writes nothing (Helpers.hs write).
+ * counts (2 entries, may be NULL) = [i, n-i]. */
+int oracle_par_filter_u32(uint64_t n, const uint32_t* in, uint32_t* out,
+                          uint32_t* flags, uint32_t op, uint32_t a, uint32_t b,
+                          uint32_t* counts) {
+    if (pred_eval(op, a, b, 0) < 0 || (n && (!in || !out))) return -1;
+    uint64_t i_true = 0;                                     /* i = last isT */
+    for (uint64_t k = 0; k < n; k++) i_true += (uint64_t)pred_eval(op, a, b, in[k]);
+    uint64_t isT = 0, isF = 0;
+    for (uint64_t k = 0; k < n; k++) {
+        if (pred_eval(op, a, b, in[k])) out[++isT - 1] = in[k];
+        else out[i_true + ++isF - 1] = in[k];
+    }
+    if (flags) {
+        memset(flags, 0, n * sizeof(uint32_t));
+        if (0 < n) flags[0] = (uint32_t)i_true;
+        if (i_true < n) flags[i_true] = (uint32_t)(n - i_true);
+    }
+    if (counts) { counts[0] = (uint32_t)i_true; counts[1] = (uint32_t)(n - i_true); }
+    return 0;
+}
+
+/* Moves every column of element i to the stable position of class cls[i]. */
+static int soa_scatter(uint64_t n, uint32_t ncols, const void* const* cols,
+                       const uint32_t* col_bytes, void* const* out_cols,
+                       const uint8_t* cls, uint32_t m, uint32_t* counts) {
+    uint64_t* red = (uint64_t*)calloc(m, sizeof(uint64_t));
+    uint64_t* cur = (uint64_t*)calloc(m, sizeof(uint64_t));
+    if (!red || !cur) { free(red); free(cur); return -2; }
+    for (uint64_t i = 0; i < n; i++) red[cls[i]]++;
+    uint64_t acc = 0;
+    for (uint32_t k = 0; k < m; k++) { cur[k] = acc; acc += red[k]; }
+    for (uint64_t i = 0; i < n; i++) {
+        uint64_t d = cur[cls[i]]++;
+        for (uint32_t j = 0; j < ncols; j++)
+            copy_elem(out_cols[j], d, cols[j], i, col_bytes[j]);
+    }
+    if (counts)
+        for (uint32_t k = 0; k < m; k++) counts[k] = (uint32_t)red[k];
+    free(red); free(cur);
+    return 0;
+}
+
+static int soa_cols_ok(uint64_t n, uint32_t ncols, const void* const* cols,
+                       const uint32_t* col_bytes, void* const* out_cols) {
+    if (ncols == 0 || !cols || !col_bytes || !out_cols) return 0;
+    for (uint32_t j = 0; j < ncols; j++) {
+        uint32_t b = col_bytes[j];
+        if (b != 1 && b != 2 && b != 4 && b != 8) return 0;
+        if (n && (!cols[j] || !out_cols[j])) return 0;
+    }
+    return 1;
+}
+
+static inline uint32_t col_word(const void* col, uint32_t bytes, uint64_t i) {
+    return bytes == 1 ? (uint32_t)((const uint8_t*)col)[i] : ((const uint32_t*)col)[i];
+}
+
+/* The two-way filter over records: class 0 where the predicate holds on the
+ * 32-bit key column, 1 elsewhere; every column moves. */
+int oracle_filter_soa(uint64_t n, uint32_t ncols, const void* const* cols,
+                      const uint32_t* col_bytes, uint32_t key_col,
+                      void* const* out_cols, uint32_t op, uint32_t a, uint32_t b,
+                      uint32_t* counts) {
+    if (!soa_cols_ok(n, ncols, cols, col_bytes, out_cols) || key_col >= ncols ||
+        col_bytes[key_col] != 4 || pred_eval(op, a, b, 0) < 0)
+        return -1;
+    uint8_t* cls = (uint8_t*)malloc(n ? n : 1);
+    if (!cls) return -2;
+    for (uint64_t i = 0; i < n; i++)
+        cls[i] = pred_eval(op, a, b, ((const uint32_t*)cols[key_col])[i]) ? 0 : 1;
+    int rc = soa_scatter(n, ncols, cols, col_bytes, out_cols, cls, 2, counts);
+    free(cls);
+    return rc;
+}
+
+/* The multi-column discriminator of MdiscrHost_tuples.cu.h:27,46
+ * (z ? x % N : y % N): class = cols[sel] != 0 ? D(cols[a]) : D(cols[b]); the
+ * three columns hold 1- or 4-byte values. */
+int oracle_seq_mdiscr_soa_select(uint64_t n, uint32_t ncols, const void* const* cols,
+                                 const uint32_t* col_bytes, uint32_t sel_col,
+                                 uint32_t a_col, uint32_t b_col, void* const* out_cols,
+                                 uint32_t kind, uint32_t m, uint32_t* counts) {
+    if (!discr_ok(kind, m) || !soa_cols_ok(n, ncols, cols, col_bytes, out_cols) ||
+        sel_col >= ncols || a_col >= ncols || b_col >= ncols)
+        return -1;
+    const uint32_t sb = col_bytes[sel_col], ab = col_bytes[a_col], bb = col_bytes[b_col];
+    if ((sb != 1 && sb != 4) || (ab != 1 && ab != 4) || (bb != 1 && bb != 4)) return -1;
+    uint8_t* cls = (uint8_t*)malloc(n ? n : 1);
+    if (!cls) return -2;
+    for (uint64_t i = 0; i < n; i++)
+        cls[i] = (uint8_t)(col_word(cols[sel_col], sb, i)
+                               ? discr(col_word(cols[a_col], ab, i), kind, m)
+                               : discr(col_word(cols[b_col], bb, i), kind, m));
+    int rc = soa_scatter(n, ncols, cols, col_bytes, out_cols, cls, m, counts);
+    free(cls);
+    return rc;
+}
+
 /* MdiscrSegmTest.cu:21-38: the segmented result is the plain result of each
  * segment placed at that segment's range. */
 int oracle_seq_mdiscr_segm(uint64_t n, const uint32_t* in,

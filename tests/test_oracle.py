@@ -246,3 +246,134 @@ def test_soa_oracle_one_byte_class_key():
     assert np.array_equal(outs[0], pay[perm])
     assert np.array_equal(outs[1], cls[perm])
     assert np.array_equal(cnt, np.bincount(cls, minlength=m))
+
+
+# ---- two-way filter (ParFilter.hs:8-21) and the select discriminator ---------
+
+def numpy_pred(x, op, a, b):
+    """Each predicate stated with numpy's own comparisons on uint64 copies."""
+    x = x.astype(np.uint64)
+    a, b = np.uint64(a), np.uint64(b)
+    return {oracle.PRED_LT: x < a, oracle.PRED_LE: x <= a, oracle.PRED_GT: x > a,
+            oracle.PRED_GE: x >= a, oracle.PRED_EQ: x == a, oracle.PRED_NE: x != a,
+            oracle.PRED_BITS: (x & a) == b, oracle.PRED_RANGE: (a <= x) & (x < b)}[op]
+
+
+PRED_CASES = [
+    (oracle.PRED_LT, 1 << 30, 0), (oracle.PRED_LT, 0, 0), (oracle.PRED_LE, 2**32 - 1, 0),
+    (oracle.PRED_LE, 12345, 0), (oracle.PRED_GT, 2**32 - 1, 0), (oracle.PRED_GT, 1 << 29, 0),
+    (oracle.PRED_GE, 0, 0), (oracle.PRED_GE, 3 << 29, 0), (oracle.PRED_EQ, 7, 0),
+    (oracle.PRED_NE, 7, 0), (oracle.PRED_BITS, 1, 0), (oracle.PRED_BITS, 0xF0, 0x30),
+    (oracle.PRED_BITS, 0x0F, 0x30), (oracle.PRED_RANGE, 1 << 28, 1 << 30),
+    (oracle.PRED_RANGE, 100, 100), (oracle.PRED_RANGE, 9, 3),
+]
+
+
+def test_par_filter_oracle_hand_example():
+    """Worked by hand from ParFilter.hs:8-21 and Helpers.hs write/permute with
+    cond = even: isT = [0,1,2,2,3], i = 3, isF = [4,4,4,5,5],
+    inds = [3,0,1,4,2], flags = write [0,3] [3,2] (replicate 5 0)."""
+    x = np.array([5, 2, 8, 3, 4], dtype=np.uint32)
+    out, flags, cnt = oracle.par_filter(x, oracle.PRED_BITS, 1, 0)
+    assert out.tolist() == [2, 8, 4, 5, 3]
+    assert flags.tolist() == [3, 0, 0, 2, 0]
+    assert cnt.tolist() == [3, 2]
+    # i = n: the write at index n matches no position.
+    out, flags, _ = oracle.par_filter(np.array([2, 4], dtype=np.uint32), oracle.PRED_BITS, 1, 0)
+    assert out.tolist() == [2, 4] and flags.tolist() == [2, 0]
+    # i = 0: write [0,0] [0,2] -- the later write to position 0 wins.
+    out, flags, _ = oracle.par_filter(np.array([1, 3], dtype=np.uint32), oracle.PRED_BITS, 1, 0)
+    assert out.tolist() == [1, 3] and flags.tolist() == [2, 0]
+    out, flags, cnt = oracle.par_filter(np.array([], dtype=np.uint32), oracle.PRED_EQ, 0)
+    assert len(out) == 0 and len(flags) == 0 and cnt.tolist() == [0, 0]
+
+
+@pytest.mark.parametrize("op,a,b", PRED_CASES)
+def test_par_filter_oracle_closed_form(op, a, b):
+    x = W.uniform_u31(20000, 31 + op) * np.uint32(2) + (W.uniform_u31(20000, 7) & 1)
+    x[:4] = [0, 7, 2**32 - 1, 0x30]
+    p = numpy_pred(x, op, a, b)
+    out, flags, cnt = oracle.par_filter(x, op, a, b)
+    assert np.array_equal(out, np.concatenate([x[p], x[~p]]))
+    i = int(p.sum())
+    assert cnt.tolist() == [i, len(x) - i]
+    assert np.array_equal(flags, oracle.sizes_from_counts(cnt, len(x)))
+
+
+@needs_ref
+def test_par_filter_flags_are_reference_size_arrays():
+    """flags = write [0,i] [i,n-i] equals the reference's tupleToSizeArray of [i, n-i]."""
+    for x in (W.uniform_u31(1000, 3), np.zeros(17, dtype=np.uint32), np.ones(9, dtype=np.uint32)):
+        _, flags, cnt = oracle.par_filter(x, oracle.PRED_BITS, 1, 0)
+        assert np.array_equal(flags, oracle.ref_tuple_to_size_array(cnt, len(x)))
+
+
+def test_par_filter_brute_force_tiny():
+    """Every keep/drop pattern of length <= 8 (values encode position and bit)."""
+    for n in range(0, 9):
+        for bits in itertools.product((0, 1), repeat=n):
+            x = np.array([2 * i + b for i, b in enumerate(bits)], dtype=np.uint32)
+            out, flags, cnt = oracle.par_filter(x, oracle.PRED_BITS, 1, 1)
+            keep = [v for v in x.tolist() if v & 1]
+            assert out.tolist() == keep + [v for v in x.tolist() if not v & 1]
+            expect = [0] * n
+            if n:
+                expect[0] = len(keep)
+            if len(keep) < n:
+                expect[len(keep)] = n - len(keep)
+            assert flags.tolist() == expect
+
+
+def test_par_filter_planted_mistakes_are_caught():
+    x = W.uniform_u31(4096, 77)
+    p = numpy_pred(x, oracle.PRED_LT, 1 << 30, 0)
+    out, _, cnt = oracle.par_filter(x, oracle.PRED_LT, 1 << 30)
+    cf = np.concatenate([x[p], x[~p]])
+    assert np.array_equal(out, cf)
+    bad = np.concatenate([x[~p], x[p]])                    # rejected elements first
+    assert not np.array_equal(bad, cf)
+    bad = np.concatenate([x[p][::-1], x[~p]])              # unstable kept order
+    assert not np.array_equal(bad, cf)
+    # an off-by-one bound (<= for <) keeps the element equal to the bound
+    x2 = x.copy()
+    x2[0] = 1 << 30
+    le = numpy_pred(x2, oracle.PRED_LE, 1 << 30, 0)
+    assert not np.array_equal(oracle.par_filter(x2, oracle.PRED_LT, 1 << 30)[0],
+                              np.concatenate([x2[le], x2[~le]]))
+
+
+def test_filter_soa_oracle():
+    n = 30000
+    key = W.uniform_u31(n, 8)
+    pay = W.splitmix64(n, 9)
+    b8 = (key & 0xFF).astype(np.uint8)
+    p = numpy_pred(key, oracle.PRED_RANGE, 1 << 29, 3 << 29)
+    outs, cnt = oracle.filter_soa([pay, key, b8], oracle.PRED_RANGE, 1 << 29, 3 << 29, key_col=1)
+    perm = np.concatenate([np.flatnonzero(p), np.flatnonzero(~p)])
+    for o, c in zip(outs, [pay, key, b8]):
+        assert np.array_equal(o, c[perm])
+    assert cnt.tolist() == [int(p.sum()), n - int(p.sum())]
+
+
+@pytest.mark.parametrize("m", [4, 5, 8, 64])
+def test_select_oracle_closed_form(m):
+    """z ? x % N : y % N over the reference's tuple (uint x, uchar y, bool z)."""
+    n = 25000
+    x = W.uniform_u31(n, m)
+    y = (W.uniform_u31(n, m + 1) & 0xFF).astype(np.uint8)
+    z = (W.uniform_u31(n, m + 2) & 1).astype(np.uint8)
+    outs, cnt = oracle.seq_mdiscr_soa_select([x, y, z], m, sel_col=2, a_col=0, b_col=1)
+    cls = np.where(z != 0, x % m, y.astype(np.uint32) % m)
+    perm = np.argsort(cls, kind="stable")
+    for o, c in zip(outs, [x, y, z]):
+        assert np.array_equal(o, c[perm])
+    assert np.array_equal(cnt, np.bincount(cls, minlength=m))
+    # all-true / all-false selectors reduce to the plain SoA partition on one key
+    ones = np.ones(n, dtype=np.uint8)
+    o1, _ = oracle.seq_mdiscr_soa_select([x, y, ones], m, 2, 0, 1)
+    r1, _ = oracle.seq_mdiscr_soa([x, y, ones], m, key_col=0)
+    assert all(np.array_equal(a, b) for a, b in zip(o1, r1))
+    zeros = np.zeros(n, dtype=np.uint8)
+    o0, _ = oracle.seq_mdiscr_soa_select([x, y, zeros], m, 2, 0, 1)
+    r0, _ = oracle.seq_mdiscr_soa([x, y, zeros], m, key_col=1)
+    assert all(np.array_equal(a, b) for a, b in zip(o0, r0))
