@@ -6,6 +6,12 @@
 //   MyInt<N> counts tuple           HelpersCommon.h:855-965 (cardinal, operator[])
 //   mdiscr<DISCR>(n, num_hwd_thds, d_in, d_out) -> DISCR::TupleType
 //                                   MdiscrHost_optimized_clean.cu.h:9-151
+//   ModArray4<LEN>, ModArray<N,LEN> MdiscrHost_arrays.cu.h:14-36
+//   mdiscr<DISCR,LEN>(n, num_hwd_thds, d_in, d_out) -> DISCR::TupleType
+//                                   MdiscrHost_arrays.cu.h:250-470
+//   ModTuple4, ModTuple<N>          MdiscrHost_tuples.cu.h:13-48
+//   mdiscr<DISCR>(n, num_hwd_thds, d_in1, d_in2, d_in3, d_out1, d_out2, d_out3)
+//                                   MdiscrHost_tuples.cu.h:262-429
 //   mdiscrSegm<DISCR>(n, in, segment_sizes, out, new_sizes)
 //                                   MdiscrSegmHost.cu.h:36-230
 //   tupleToSizeArray(sizes, sizes_array, n)
@@ -23,6 +29,8 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+
+#include <hip/hip_runtime_api.h>
 
 #include "mdiscr.h"
 
@@ -90,6 +98,27 @@ inline mdiscr_ctx* context() {
     return ctx;
 }
 
+// Device memory for the class counts of the calls whose C entry point writes
+// them to the device (64 entries, allocated on first use, kept for the process).
+inline uint32_t* device_counts() {
+    static uint32_t* dc = [] {
+        void* p = nullptr;
+        if (hipMalloc(&p, MDISCR_MAX_CLASSES * sizeof(uint32_t)) != hipSuccess)
+            fail(MDISCR_ENOMEM, "counts buffer", nullptr);
+        return static_cast<uint32_t*>(p);
+    }();
+    return dc;
+}
+
+// Waits for the call on the null stream and copies the m counts to the host.
+inline void read_counts(mdiscr_ctx* ctx, unsigned int* h, unsigned int m, const char* what) {
+    mdiscr_status s = mdiscr_sync(ctx, nullptr);
+    if (s == MDISCR_OK &&
+        hipMemcpy(h, device_counts(), m * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess)
+        s = MDISCR_EDEVICE;
+    if (s != MDISCR_OK) fail(s, what, ctx);
+}
+
 }  // namespace detail
 
 // mdiscr<DISCR>(num_elems, num_hwd_thds, d_in, d_out): stable partition of
@@ -105,6 +134,105 @@ typename DISCR::TupleType mdiscr(const unsigned int num_elems, const unsigned in
     if (s != MDISCR_OK) detail::fail(s, "mdiscr", ctx);
     return counts;
 }
+
+// ---- array-valued elements ----------------------------------------------
+
+// Elements are LEN consecutive unsigned ints; the class comes from word 0.
+template <unsigned int LEN>
+struct ModArray4 {
+    typedef unsigned int InType;
+    typedef MyInt<4> TupleType;
+    static const InType padelm = 3;
+    static unsigned int apply(const InType* x) { return x[0] & 3; }
+    static mdiscr_discr discr() { return mdiscr_discr{MDISCR_DISCR_MASK, 4}; }
+};
+
+template <unsigned int N, unsigned int LEN>
+struct ModArray {
+    static_assert(N >= 1 && N <= MDISCR_MAX_CLASSES, "1 <= N <= 64");
+    typedef unsigned int InType;
+    typedef MyInt<N> TupleType;
+    static const InType padelm = N - 1;
+    static unsigned int apply(const InType* x) { return x[0] % N; }
+    static mdiscr_discr discr() { return mdiscr_discr{MDISCR_DISCR_MOD, N}; }
+};
+
+// mdiscr<DISCR, LEN>(num_elems, num_hwd_thds, d_in, d_out): num_elems arrays
+// of LEN words each, moved whole to their stable class positions.
+template <class DISCR, unsigned int LEN>
+typename DISCR::TupleType mdiscr(const unsigned int num_elems, const unsigned int num_hwd_thds,
+                                 typename DISCR::InType* d_in, typename DISCR::InType* d_out) {
+    (void)num_hwd_thds;
+    typename DISCR::TupleType counts;
+    mdiscr_ctx* ctx = detail::context();
+    const mdiscr_status s = mdiscr_partition_arrays(ctx, d_in, d_out, num_elems, LEN,
+                                                    DISCR::discr(), detail::device_counts(),
+                                                    nullptr);
+    if (s != MDISCR_OK) detail::fail(s, "mdiscr (arrays)", ctx);
+    detail::read_counts(ctx, counts.arr, DISCR::TupleType::cardinal, "mdiscr (arrays)");
+    return counts;
+}
+
+// ---- three-column tuples -------------------------------------------------
+
+// (unsigned int, unsigned char, bool) records classified by the first column.
+struct ModTuple4 {
+    typedef unsigned int InType1;
+    typedef unsigned char InType2;
+    typedef bool InType3;
+    typedef MyInt<4> TupleType;
+    static const bool select = false;
+    static unsigned int apply(InType1 x, InType2, InType3) { return x & 3; }
+    static mdiscr_discr discr() { return mdiscr_discr{MDISCR_DISCR_MASK, 4}; }
+};
+
+template <unsigned int N>
+struct ModTuple {
+    static_assert(N >= 1 && N <= MDISCR_MAX_CLASSES, "1 <= N <= 64");
+    typedef unsigned int InType1;
+    typedef unsigned char InType2;
+    typedef bool InType3;
+    typedef MyInt<N> TupleType;
+    static const bool select = false;
+    static unsigned int apply(InType1 x, InType2, InType3) { return x % N; }
+    static mdiscr_discr discr() { return mdiscr_discr{MDISCR_DISCR_MOD, N}; }
+};
+
+// The alternative body the reference keeps commented out in ModTuple4 and
+// ModTuple<N> (MdiscrHost_tuples.cu.h:27,46): z ? x % N : y % N.
+template <unsigned int N>
+struct ModTupleSelect : ModTuple<N> {
+    static const bool select = true;
+    static unsigned int apply(unsigned int x, unsigned char y, bool z) { return z ? x % N : y % N; }
+};
+
+// mdiscr<DISCR>(num_elems, num_hwd_thds, d_in1..3, d_out1..3): the three
+// columns move together; the class is DISCR::apply of the record.
+template <class DISCR>
+typename DISCR::TupleType mdiscr(const unsigned int num_elems, const unsigned int num_hwd_thds,
+                                 typename DISCR::InType1* d_in1, typename DISCR::InType2* d_in2,
+                                 typename DISCR::InType3* d_in3, typename DISCR::InType1* d_out1,
+                                 typename DISCR::InType2* d_out2, typename DISCR::InType3* d_out3) {
+    (void)num_hwd_thds;
+    static_assert(sizeof(typename DISCR::InType3) == 1, "bool is one byte here");
+    typename DISCR::TupleType counts;
+    mdiscr_ctx* ctx = detail::context();
+    const void* in[3] = {d_in1, d_in2, d_in3};
+    void* out[3] = {d_out1, d_out2, d_out3};
+    const uint32_t bytes[3] = {sizeof(typename DISCR::InType1), sizeof(typename DISCR::InType2),
+                               sizeof(typename DISCR::InType3)};
+    const mdiscr_status s =
+        DISCR::select
+            ? mdiscr_partition_soa_select(ctx, 3, in, bytes, 2, 0, 1, out, num_elems,
+                                          DISCR::discr(), detail::device_counts(), nullptr)
+            : mdiscr_partition_soa(ctx, 3, in, bytes, 0, out, num_elems, DISCR::discr(),
+                                   detail::device_counts(), nullptr);
+    if (s != MDISCR_OK) detail::fail(s, "mdiscr (tuples)", ctx);
+    detail::read_counts(ctx, counts.arr, DISCR::TupleType::cardinal, "mdiscr (tuples)");
+    return counts;
+}
+
+// ---- segmented -------------------------------------------------------------
 
 // mdiscrSegm<DISCR>(num_elems, in_array, segment_sizes, out_array, new_sizes):
 // every flag-delimited segment partitioned in place of its range; new_sizes

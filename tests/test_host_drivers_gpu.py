@@ -22,7 +22,10 @@ def run(args, timeout=120):
 
 @pytest.mark.parametrize("n,m,mode", [(1000003, 4, ""), (1000003, 8, ""), (77777, 3, ""),
                                       (4096, 64, ""), (1, 5, ""), (300000, 16, "segm"),
-                                      (123457, 4, "segm")])
+                                      (123457, 4, "segm"), (100003, 4, "arrays"),
+                                      (77777, 5, "arrays"), (4097, 64, "arrays"),
+                                      (1000003, 4, "tuples"), (65536, 8, "tuples"),
+                                      (300007, 64, "select"), (100003, 5, "select")])
 def test_reference_shaped_driver(n, m, mode):
     r = run([REF_DRIVER, n, m] + ([mode] if mode else []))
     assert r.returncode == 0, r.stdout + r.stderr
