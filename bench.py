@@ -219,6 +219,45 @@ def main():
                 "kernels_ms": {k: round(v, 5) for k, v in kts.items()}}
         del k_in, p_in, i_in, outs
 
+    # The other element shapes and discriminators, each one call on resident
+    # data: parFilter on the 2^28 keys, 2^26 arrays of 4 words (class from word
+    # 0), and 2^26 (uint x, uchar y, bool z) records under z ? x % 8 : y % 8.
+    general = {}
+    if not args.no_drivers:
+        def shape_line(fn, n_units, bytes_per_unit, names, part_name):
+            dt_g = time_steps(ctx, fn, 10, 2)
+            ktg = kernel_times(ctx, fn, 10, names)
+            ms_g = dt_g / 10 * 1e3
+            return {"ms_per_call": round(ms_g, 4), "g_units_s": round(n_units / ms_g / 1e6, 2),
+                    "hbm_frac_whole_path": round(n_units * bytes_per_unit / (ms_g * 1e-3) / 1e9
+                                                 / PEAK_HBM_GBS, 4),
+                    "partition_frac": round(n_units * bytes_per_unit
+                                            / (ktg[part_name] * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                    "bytes_per_unit_moved": bytes_per_unit,
+                    "kernels_ms": {k: round(v, 5) for k, v in ktg.items()}}
+
+        fc = torch.empty(2, dtype=torch.int32, device="cuda:0")
+        general["filter_2p28_lt_2p30"] = shape_line(
+            lambda: ctx.filter_u32(d_in, d_out, md.Pred.lt(1 << 30), d_counts=fc, stream=stream),
+            N, 8, kernels, "partition")
+        na, length = 1 << 26, 4
+        a_in = W.uniform_u31_torch(na * length, SEED + 2, "cuda:0")
+        a_out = torch.empty_like(a_in)
+        general["arrays_2p26_len4_m16"] = shape_line(
+            lambda: ctx.partition_arrays(a_in, a_out, length, md.Mask(16), stream=stream),
+            na, 2 * 4 * length, ["soa_count", "soa_scan", "soa_partition"], "soa_partition")
+        del a_in, a_out
+        nt = 1 << 26
+        tx = W.uniform_u31_torch(nt, SEED + 3, "cuda:0")
+        ty = (W.uniform_u31_torch(nt, SEED + 4, "cuda:0") & 0xFF).to(torch.uint8)
+        tz = (W.uniform_u31_torch(nt, SEED + 5, "cuda:0") & 1).to(torch.uint8)
+        touts = [torch.empty_like(t) for t in (tx, ty, tz)]
+        general["select_tuples_2p26_m8"] = shape_line(
+            lambda: ctx.partition_soa_select([tx, ty, tz], touts, md.Mask(8), 2, 0, 1,
+                                             stream=stream),
+            nt, 2 * 6, ["soa_count", "soa_scan", "soa_partition"], "soa_partition")
+        del tx, ty, tz, touts
+
     seg = serve = None
     if not args.no_drivers:
         log("batched segmented vs one plain launch per segment (C driver)")
@@ -258,6 +297,7 @@ def main():
         "zipf_variant": {"value": round(N * args.steps / dtz / 1e9, 3), "unit": "Gelem/s",
                          "kernels_ms": {k: (round(v, 5) if v else v) for k, v in ktz.items()}},
         "soa_records_2p26": soa or None,
+        "general": general or None,
         "batched_segments": seg,
         "serving": serve,
         "cpu_baseline": cpu,

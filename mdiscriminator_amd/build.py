@@ -14,6 +14,7 @@ LIB = os.path.join(PKG, "libmdiscr.so")
 SOURCES = [
     "mdiscr_api.hip",
     "soa.hip",
+    "arrays.hip",
     "segmented.hip",
     "serve.cpp",
 ]

@@ -273,30 +273,3 @@ extern "C" mdiscr_status mdiscr_partition_soa_select(mdiscr_ctx* ctx, uint32_t n
     cs.b = b_col;
     return run_soa(ctx, cols, n, cs, d_counts, stream);
 }
-
-extern "C" mdiscr_status mdiscr_partition_arrays(mdiscr_ctx* ctx, const uint32_t* d_in,
-                                                 uint32_t* d_out, uint64_t n, uint32_t len,
-                                                 mdiscr_discr d, uint32_t* d_counts,
-                                                 void* stream) {
-    if (!ctx) return MDISCR_EINVAL;
-    mdiscr_status st = check_discr(ctx, d);
-    if (st) return st;
-    if (len == 0 || len > (uint32_t)kMaxCols)
-        return ctx_fail(ctx, MDISCR_EINVAL, "array length %u outside [1,%d]", len, kMaxCols);
-    if (n > MDISCR_MAX_ELEMS) return ctx_fail(ctx, MDISCR_EINVAL, "n too large");
-    if (n && (!d_in || !d_out)) return ctx_fail(ctx, MDISCR_EINVAL, "null data pointer");
-    if (n && ranges_overlap(d_in, d_out, n * len * 4))
-        return ctx_fail(ctx, MDISCR_EINVAL, "output overlaps input");
-    SoaCols cols = {};
-    cols.ncols = len;
-    cols.key = 0;  // class from element word 0 (ModArray, MdiscrHost_arrays.cu.h:27-36)
-    for (uint32_t j = 0; j < len; ++j) {
-        cols.in[j] = d_in + j;
-        cols.out[j] = d_out + j;
-        cols.bytes[j] = 4;
-        cols.stride[j] = len;
-    }
-    ClassSpec cs;
-    cs.d = d;
-    return run_soa(ctx, cols, n, cs, d_counts, stream);
-}

@@ -4,9 +4,7 @@
 // 262-429, MdiscrKernels_tuples.cu.h:4-236): the class comes from one key
 // column (ModTuple<N>::apply, MdiscrHost_tuples.cu.h:44-45) and every column
 // moves to the same destination, as raw bits.  Columns carry an element
-// stride, so fixed-length array elements stored flat (the reference's array
-// variant, MdiscrHost_arrays.cu.h:14-96: class from element word 0, the whole
-// element moved) are LEN strided 32-bit columns with word 0 as the key.  A
+// stride (the count pass over array elements reads word 0 at stride LEN).  A
 // 1-byte key column lets a caller-computed class array be the discriminator;
 // a select source reads the class from one of two columns, chosen per element
 // by a third (the tuple discriminator's z ? x % N : y % N form).

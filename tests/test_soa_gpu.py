@@ -138,12 +138,14 @@ def test_soa_invalid(ctx):
         ctx.partition_soa([a], [a], md.Mod4)                                        # aliasing
 
 
-@pytest.mark.parametrize("length", [1, 2, 3, 4, 5, 8])
+@pytest.mark.parametrize("length", [1, 2, 3, 4, 5, 6, 7, 8])
 def test_array_elements(ctx, length):
-    """Array-valued elements (mdiscr<DISCR,LEN>, MdiscrHost_arrays.cu.h:250-470)."""
+    """Array-valued elements (mdiscr<DISCR,LEN>, MdiscrHost_arrays.cu.h:250-470),
+    around the row kernel's tile sizes (1024, 2048 and 4096 elements)."""
     import torch
     import mdiscriminator_amd as md
-    for n, m in ((1, 4), (8191, 8), (100003, 5), (1 << 18, 64)):
+    for n, m in ((1, 4), (1023, 2), (1025, 3), (2049, 1), (4097, 16), (8191, 8), (100003, 5),
+                 (1 << 18, 64)):
         x = W.uniform_u31(n * length, 900 + length + n)
         d_in = torch.from_numpy(x.view(np.int32)).to("cuda:0")
         d_out = torch.full_like(d_in, -1)
