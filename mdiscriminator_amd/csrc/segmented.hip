@@ -26,6 +26,7 @@ using namespace mdiscr;
 namespace {
 
 constexpr int kPlanThreads = 1024;
+constexpr uint32_t kExpandGrid = 1024;  // workgroups writing the T items of long segments
 
 struct SegConfig {
     int nw, r;
@@ -48,24 +49,55 @@ constexpr uint32_t kMinTile = 1024;  // smallest NW*64*R among the candidates
 struct SegArgs {
     const SegDesc* desc;
     SegItem* items;
+    SegLong* longs;
     SegPlan* plan;
     uint32_t* counts;
     uint32_t ld;
     uint32_t nseg;
     uint32_t mrows;
-    uint32_t grid;
+    uint32_t grid_cap;  // no more workgroups than the queue can hold
+    uint32_t num_cu;
 };
+
+// Workgroups of `kernel` resident on one CU (the occupancy API), at least 1.
+template <typename K>
+int resident_per_cu(K kernel, int threads) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, threads, 0) != hipSuccess ||
+        nb < 1)
+        nb = 1;
+    return nb;
+}
+
+#ifndef MDISCR_SEG_OVERSUB
+#define MDISCR_SEG_OVERSUB 16
+#endif
+// The queue-walking grid: MDISCR_SEG_OVERSUB times what fits at once, so
+// each workgroup takes a few items and finished workgroups are replaced by
+// the hardware (a grid of exactly the resident capacity leaves the last
+// items' imbalance as a tail), and never more workgroups than items.
+inline uint32_t persistent_grid(int per_cu, const SegArgs& a) {
+    uint64_t g = (uint64_t)per_cu * a.num_cu * MDISCR_SEG_OVERSUB;
+    if (g > a.grid_cap) g = a.grid_cap;
+    return (uint32_t)(g ? g : 1);
+}
 
 template <int NW, int R, int NBITS, bool MASK, bool STAGED>
 void launch_cfg(const SegArgs& a, hipStream_t s) {
     hipLaunchKernelGGL((k_seg_plan<kPlanThreads, NW, R>), dim3(1), dim3(kPlanThreads), 0, s,
-                       a.desc, a.nseg, a.items, a.plan, a.counts, a.ld, a.mrows);
-    hipLaunchKernelGGL((k_seg_count<NW, R, NBITS, MASK>), dim3(a.grid), dim3(NW * kWave), 0, s,
-                       a.desc, a.items, a.plan, a.counts, a.ld);
+                       a.desc, a.nseg, a.items, a.longs, a.plan, a.counts, a.ld, a.mrows);
+    const uint32_t eg = a.nseg < kExpandGrid ? (a.nseg ? a.nseg : 1u) : kExpandGrid;
+    hipLaunchKernelGGL((k_seg_expand<256>), dim3(eg), dim3(256), 0, s, a.longs, a.plan, a.items);
+    static const int count_per_cu = resident_per_cu(k_seg_count<NW, R, NBITS, MASK>, NW * kWave);
+    static const int part_per_cu =
+        resident_per_cu(k_seg_partition<NW, R, NBITS, MASK, STAGED>, NW * kWave);
+    hipLaunchKernelGGL((k_seg_count<NW, R, NBITS, MASK>), dim3(persistent_grid(count_per_cu, a)),
+                       dim3(NW * kWave), 0, s, a.desc, a.items, a.plan, a.counts, a.ld);
     hipLaunchKernelGGL((k_tile_scan<1024, 64>), dim3(a.mrows), dim3(1024), 0, s, a.counts, 0u,
                        a.ld, (uint32_t*)nullptr, (uint32_t*)nullptr, &a.plan->n_rows);
-    hipLaunchKernelGGL((k_seg_partition<NW, R, NBITS, MASK, STAGED>), dim3(a.grid),
-                       dim3(NW * kWave), 0, s, a.desc, a.items, a.plan, a.counts, a.ld);
+    hipLaunchKernelGGL((k_seg_partition<NW, R, NBITS, MASK, STAGED>),
+                       dim3(persistent_grid(part_per_cu, a)), dim3(NW * kWave), 0, s, a.desc,
+                       a.items, a.plan, a.counts, a.ld);
 }
 
 template <int NBITS, bool MASK>
@@ -194,6 +226,7 @@ void tune_save(SegState* st) {
 struct SegWorkspace {
     SegDesc* desc;
     SegItem* items;
+    SegLong* longs;
     SegPlan* plan;
     uint32_t* counts;
     uint32_t ld;
@@ -213,6 +246,7 @@ mdiscr_status seg_workspace(mdiscr_ctx* ctx, uint64_t n, uint64_t nseg, uint32_t
     };
     const size_t o_desc = take(nseg * sizeof(SegDesc));
     const size_t o_items = take(max_items * sizeof(SegItem));
+    const size_t o_longs = take(nseg * sizeof(SegLong));
     const size_t o_plan = take(sizeof(SegPlan));
     const size_t o_counts = take((size_t)m * ld * sizeof(uint32_t));
     void* base = nullptr;
@@ -221,19 +255,15 @@ mdiscr_status seg_workspace(mdiscr_ctx* ctx, uint64_t n, uint64_t nseg, uint32_t
     char* b = static_cast<char*>(base);
     w.desc = reinterpret_cast<SegDesc*>(b + o_desc);
     w.items = reinterpret_cast<SegItem*>(b + o_items);
+    w.longs = reinterpret_cast<SegLong*>(b + o_longs);
     w.plan = reinterpret_cast<SegPlan*>(b + o_plan);
     w.counts = reinterpret_cast<uint32_t*>(b + o_counts);
     w.ld = (uint32_t)ld;
     return MDISCR_OK;
 }
 
-uint32_t persistent_grid(const mdiscr_ctx* ctx, int cfg, uint64_t max_items) {
-    // Workgroups resident at once, with headroom: enough to fill every CU.
-    const int nw = kConfigs[cfg].nw;
-    const uint64_t per_cu = nw >= 16 ? 2 : (nw >= 8 ? 4 : 8);
-    uint64_t g = (uint64_t)ctx->num_cu * per_cu;
-    if (g > max_items) g = max_items;
-    return (uint32_t)(g ? g : 1);
+uint32_t grid_cap(uint64_t max_items) {
+    return (uint32_t)(max_items > 0xFFFFFFFFull ? 0xFFFFFFFFull : (max_items ? max_items : 1));
 }
 
 mdiscr_status run_segmented(mdiscr_ctx* ctx, const uint32_t* d_in, uint32_t* d_out, uint64_t n,
@@ -247,12 +277,14 @@ mdiscr_status run_segmented(mdiscr_ctx* ctx, const uint32_t* d_in, uint32_t* d_o
     SegArgs a;
     a.desc = w.desc;
     a.items = w.items;
+    a.longs = w.longs;
     a.plan = w.plan;
     a.counts = w.counts;
     a.ld = w.ld;
     a.nseg = (uint32_t)nseg;
     a.mrows = d.m;
-    a.grid = persistent_grid(ctx, cfg, 2 * nseg + n / kMinTile + 2);
+    a.grid_cap = grid_cap(2 * nseg + n / kMinTile + 2);
+    a.num_cu = (uint32_t)ctx->num_cu;
     const mdiscr_divmagic g = mdiscr_divmagic_make(d.m);
 
     auto issue = [&](hipStream_t q) -> bool {
@@ -608,6 +640,7 @@ mdiscr_status mdiscr_partition_batch(mdiscr_ctx* ctx, const mdiscr_request* reqs
     SegArgs a;
     a.desc = w.desc;
     a.items = w.items;
+    a.longs = w.longs;
     a.plan = w.plan;
     a.counts = w.counts;
     a.ld = w.ld;
@@ -618,7 +651,8 @@ mdiscr_status mdiscr_partition_batch(mdiscr_ctx* ctx, const mdiscr_request* reqs
         const uint64_t wt = (uint64_t)kWave * kConfigs[cfg].r;
         uint64_t est = 0;
         for (uint32_t i = 0; i < nreq; ++i) est += reqs[i].n <= wt ? 1 : (reqs[i].n + tile - 1) / tile;
-        a.grid = persistent_grid(ctx, cfg, est);
+        a.grid_cap = grid_cap(est);
+        a.num_cu = (uint32_t)ctx->num_cu;
     }
     TimedLaunch tl;
     timing_begin(ctx, s, tl);

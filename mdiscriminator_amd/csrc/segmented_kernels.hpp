@@ -44,6 +44,16 @@ struct SegItem {
 struct SegPlan {
     uint32_t n_items;
     uint32_t n_rows;
+    uint32_t n_long;  // multi-tile segments, expanded into T items by k_seg_expand
+};
+
+// A multi-tile segment whose T items the planner reserved but did not write:
+// items[item + j] = {seg, j, tiles, row + j} for j < tiles.
+struct SegLong {
+    uint32_t seg;
+    uint32_t item;
+    uint32_t tiles;
+    uint32_t row;
 };
 
 template <bool MASK>
@@ -107,24 +117,29 @@ __device__ __forceinline__ uint32_t block_excl_sum(uint32_t v, uint32_t* wsum, u
 template <int THREADS, int NW, int R>
 __global__ __launch_bounds__(THREADS) void k_seg_plan(const SegDesc* __restrict__ desc,
                                                       uint32_t nseg, SegItem* __restrict__ items,
+                                                      SegLong* __restrict__ longs,
                                                       SegPlan* __restrict__ plan,
                                                       uint32_t* __restrict__ counts, uint32_t ld,
                                                       uint32_t mrows) {
     // Segments are grouped by index, NW per group; a group's short segments
     // (len <= 64*R) form one W item whose bit w marks segment first+w.  Each
-    // thread owns SPT consecutive segments (whole groups) per pass.
+    // thread owns SPT consecutive segments (whole groups) per pass.  A
+    // one-tile segment gets its T item here; a multi-tile segment gets its
+    // items and count rows reserved and one SegLong record, and k_seg_expand
+    // writes its T items with a whole workgroup (one thread writing hundreds
+    // of items would hold up the block).
     constexpr uint32_t WT = kWave * R;          // one wave's elements
     constexpr uint32_t TILE = NW * kWave * R;   // one workgroup's elements
     constexpr uint32_t SPT = 16;
     static_assert(SPT % NW == 0, "whole groups per thread");
     __shared__ uint32_t wsum[THREADS / kWave];
-    uint32_t item_carry = 0, row_carry = 0;
+    uint32_t item_carry = 0, row_carry = 0, long_carry = 0;
     for (uint32_t base = 0; base < nseg; base += THREADS * SPT) {
         const uint32_t s0 = base + threadIdx.x * SPT;
         uint32_t len[SPT];
 #pragma unroll
         for (uint32_t k = 0; k < SPT; ++k) len[k] = (s0 + k < nseg) ? desc[s0 + k].len : 0u;
-        uint32_t nitems = 0, nrows = 0;
+        uint32_t nitems = 0, nrows = 0, nlong = 0;
 #pragma unroll
         for (uint32_t g = 0; g < SPT; g += NW) {
             bool any_short = false;
@@ -138,13 +153,15 @@ __global__ __launch_bounds__(THREADS) void k_seg_plan(const SegDesc* __restrict_
                     const uint32_t tiles = (len[k] + TILE - 1) / TILE;
                     nitems += tiles;
                     nrows += tiles > 1 ? tiles : 0u;
+                    nlong += tiles > 1 ? 1u : 0u;
                 }
             }
             nitems += any_short ? 1u : 0u;
         }
-        uint32_t itot, rtot;
+        uint32_t itot, rtot, ltot;
         uint32_t ib = item_carry + block_excl_sum<THREADS>(nitems, wsum, itot);
         uint32_t rb = row_carry + block_excl_sum<THREADS>(nrows, wsum, rtot);
+        uint32_t lb = long_carry + block_excl_sum<THREADS>(nlong, wsum, ltot);
 #pragma unroll
         for (uint32_t g = 0; g < SPT; g += NW) {
             uint32_t mask = 0;
@@ -157,18 +174,35 @@ __global__ __launch_bounds__(THREADS) void k_seg_plan(const SegDesc* __restrict_
                 const uint32_t k = g + w;
                 if (s0 + k >= nseg || len[k] <= WT) continue;
                 const uint32_t tiles = (len[k] + TILE - 1) / TILE;
-                for (uint32_t j = 0; j < tiles; ++j)
-                    items[ib + j] = SegItem{s0 + k, j, tiles, tiles > 1 ? rb + j : 0u};
+                if (tiles == 1) {
+                    items[ib] = SegItem{s0 + k, 0u, 1u, 0u};
+                } else {
+                    longs[lb++] = SegLong{s0 + k, ib, tiles, rb};
+                    rb += tiles;
+                }
                 ib += tiles;
-                rb += tiles > 1 ? tiles : 0u;
             }
         }
         item_carry += itot;
         row_carry += rtot;
+        long_carry += ltot;
     }
-    if (threadIdx.x == 0) *plan = SegPlan{item_carry, row_carry};
+    if (threadIdx.x == 0) *plan = SegPlan{item_carry, row_carry, long_carry};
     // A zero after the last row: the scan's exclusive prefix there is the total.
     for (uint32_t c = threadIdx.x; c < mrows; c += THREADS) counts[(size_t)c * ld + row_carry] = 0u;
+}
+
+// The T items of the multi-tile segments: one workgroup per SegLong record.
+template <int THREADS>
+__global__ __launch_bounds__(THREADS) void k_seg_expand(const SegLong* __restrict__ longs,
+                                                        const SegPlan* __restrict__ plan,
+                                                        SegItem* __restrict__ items) {
+    const uint32_t n_long = plan->n_long;
+    for (uint32_t k = blockIdx.x; k < n_long; k += gridDim.x) {
+        const SegLong L = longs[k];
+        for (uint32_t j = threadIdx.x; j < L.tiles; j += THREADS)
+            items[L.item + j] = SegItem{L.seg, j, L.tiles, L.row + j};
+    }
 }
 
 // ---------------------------------------------------------------------------
