@@ -234,7 +234,7 @@ def main():
                     "partition_frac": round(n_units * bytes_per_unit
                                             / (ktg[part_name] * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
                     "bytes_per_unit_moved": bytes_per_unit,
-                    "kernels_ms": {k: round(v, 5) for k, v in ktg.items()}}
+                    "kernels_ms": {k: round(v, 5) for k, v in ktg.items() if v}}
 
         fc = torch.empty(2, dtype=torch.int32, device="cuda:0")
         general["filter_2p28_lt_2p30"] = shape_line(
@@ -293,9 +293,9 @@ def main():
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": N * BYTES_PER_ELEM,
                      "avg_launch_ms": round(part_ms, 5)},
-        "kernels_ms": {k: (round(v, 5) if v else v) for k, v in kt.items()},
+        "kernels_ms": {k: round(v, 5) for k, v in kt.items() if v},
         "zipf_variant": {"value": round(N * args.steps / dtz / 1e9, 3), "unit": "Gelem/s",
-                         "kernels_ms": {k: (round(v, 5) if v else v) for k, v in ktz.items()}},
+                         "kernels_ms": {k: round(v, 5) for k, v in ktz.items() if v}},
         "soa_records_2p26": soa or None,
         "general": general or None,
         "batched_segments": seg,

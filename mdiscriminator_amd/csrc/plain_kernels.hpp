@@ -461,8 +461,8 @@ __global__ __launch_bounds__(NW* kWave, MINW) void k_partition(
 // may capture the call in a graph).
 template <int = 0>
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, uint32_t cnt) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < cnt) p[i] = v;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x)
+        p[i] = v;
 }
 
 // sizes[0,n) = 0 except sizes[off[k]] = counts[k] for non-empty classes k
