@@ -78,8 +78,15 @@ struct Geom {
 // packed in one or two 64-bit words (one shift and a 64-bit add per element);
 // wider class sets count with LDS atomics.
 template <int NBITS>
+// Packed byte counters pay a per-tile wave reduction (4 wave sums up to 8
+// classes, 8 for 16); at 16 classes LDS atomics count faster (measured: SoA
+// count over 2^26 keys 0.046 ms atomic vs 0.08 ms packed), at 8 or fewer the
+// packed form does (plain 2^28, m=8: 0.167 ms packed vs 0.198 ms atomic).
+#ifndef MDISCR_PACKED_MAX_BITS
+#define MDISCR_PACKED_MAX_BITS 3
+#endif
 struct LaneCounts {
-    static constexpr bool kPacked = NBITS <= 4;
+    static constexpr bool kPacked = NBITS <= MDISCR_PACKED_MAX_BITS;
     uint64_t lo = 0, hi = 0;
     __device__ __forceinline__ void add(uint32_t c) {
         const uint64_t inc = 1ull << ((c & 7u) * 8u);
