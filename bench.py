@@ -228,11 +228,12 @@ def main():
             dt_g = time_steps(ctx, fn, 10, 2)
             ktg = kernel_times(ctx, fn, 10, names)
             ms_g = dt_g / 10 * 1e3
+            part = ktg.get(part_name)
             return {"ms_per_call": round(ms_g, 4), "g_units_s": round(n_units / ms_g / 1e6, 2),
                     "hbm_frac_whole_path": round(n_units * bytes_per_unit / (ms_g * 1e-3) / 1e9
                                                  / PEAK_HBM_GBS, 4),
-                    "partition_frac": round(n_units * bytes_per_unit
-                                            / (ktg[part_name] * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                    "partition_frac": (round(n_units * bytes_per_unit / (part * 1e-3) / 1e9
+                                             / PEAK_HBM_GBS, 4) if part else None),
                     "bytes_per_unit_moved": bytes_per_unit,
                     "kernels_ms": {k: round(v, 5) for k, v in ktg.items() if v}}
 
@@ -245,7 +246,7 @@ def main():
         a_out = torch.empty_like(a_in)
         general["arrays_2p26_len4_m16"] = shape_line(
             lambda: ctx.partition_arrays(a_in, a_out, length, md.Mask(16), stream=stream),
-            na, 2 * 4 * length, ["soa_count", "soa_scan", "soa_partition"], "soa_partition")
+            na, 2 * 4 * length, ["rows_count", "rows_scan", "rows_partition"], "rows_partition")
         del a_in, a_out
         nt = 1 << 26
         tx = W.uniform_u31_torch(nt, SEED + 3, "cuda:0")

@@ -1,0 +1,43 @@
+"""bench.py end to end with small settings: one JSON line carrying every
+section (roofline, per-shape lines, host drivers), all measured."""
+import json
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(400)
+def test_bench_line_is_complete():
+    print("running bench.py --no-cpu --steps 2 --warmup 1 --serve-requests 500", flush=True)
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--no-cpu", "--steps", "2",
+                        "--warmup", "1", "--serve-requests", "500"],
+                       capture_output=True, text=True, timeout=380)
+    assert r.returncode == 0, r.stderr[-3000:]
+    lines = [ln for ln in r.stdout.splitlines() if ln.strip()]
+    assert len(lines) == 1, r.stdout[-2000:]
+    d = json.loads(lines[0])
+    for k in ("metric", "value", "unit", "n_gpus", "steps", "warmup", "ms_per_step", "dtype",
+              "config", "roofline", "cpu_baseline"):
+        assert k in d, k
+    assert d["n_gpus"] == 1 and d["value"] > 0
+    rf = d["roofline"]
+    assert rf["bound"] == "hbm" and 0 < rf["frac"] < 1 and rf["achieved"] > 0
+    for m in ("m16", "m32", "m64"):
+        assert d["soa_records_2p26"][m]["partition_frac"] > 0
+    for name, line in d["general"].items():
+        assert line["partition_frac"] and line["partition_frac"] > 0, name
+    assert d["batched_segments"]["identical"] is True
+    assert d["serving"]["identical_spot_checks"] is True
+
+
+def test_bench_refuses_several_gpus():
+    """Runs on the CPU: the refusal comes before anything touches a GPU."""
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
