@@ -212,13 +212,18 @@ __global__ __launch_bounds__(NW* kWave) void k_chunk_count(
 // ---------------------------------------------------------------------------
 // tile scan: for class c = blockIdx.x, row = counts + c*ld (ld a multiple of
 // 4, so rows are 16-byte aligned): row[t] <- sum_{t'<t} row[t'] for t < T,
-// totals[c] <- the class total (and user_counts[c] when given).  Each thread
-// owns PER consecutive entries per pass, fetched with PER/4 16-byte loads.
+// totals[c] <- the class total (and user_counts[c] when given).  A pass
+// covers THREADS*PER entries; wave w owns the contiguous segment of 64*PER
+// of them, and its lane l reads and writes entries 256j + 4l .. 256j + 4l + 3
+// of the segment (j < PER/4), so every load and store instruction of a wave
+// touches 1 KiB of consecutive entries.
 template <int THREADS, int PER>
 __global__ __launch_bounds__(THREADS) void k_tile_scan(
     uint32_t* __restrict__ counts, uint32_t T, uint32_t ld, uint32_t* __restrict__ totals,
     uint32_t* __restrict__ user_counts, const uint32_t* __restrict__ rows_dev = nullptr) {
     static_assert(PER % 4 == 0, "vector loads");
+    constexpr int Q = PER / 4;                   // 16-byte accesses per lane per pass
+    constexpr uint32_t WSEG = kWave * PER;       // entries per wave per pass
     // Row count from device memory (a planned queue): rows plus the zero after them.
     if (rows_dev) T = *rows_dev + 1u;
     constexpr int NWAVE = THREADS / kWave;
@@ -227,23 +232,30 @@ __global__ __launch_bounds__(THREADS) void k_tile_scan(
     const uint32_t lane = lane_id(), wave = threadIdx.x / kWave;
     uint32_t carry = 0;
     for (uint32_t pbase = 0; pbase < T; pbase += THREADS * PER) {
-        const uint32_t beg = pbase + threadIdx.x * PER;
-        uint32_t x[PER];
-        if (beg + PER <= T) {
+        const uint32_t seg = pbase + wave * WSEG;
+        u32x4 q[Q];
+        uint32_t pre[Q];  // entries of the segment before this lane's four, per access
+        uint32_t run = 0;
 #pragma unroll
-            for (int j = 0; j < PER / 4; ++j) {
-                const u32x4 q = reinterpret_cast<const u32x4*>(row + beg)[j];
-                x[4 * j] = q.x; x[4 * j + 1] = q.y; x[4 * j + 2] = q.z; x[4 * j + 3] = q.w;
+        for (int j = 0; j < Q; ++j) {
+            const uint32_t e0 = seg + j * (kWave * 4) + lane * 4;
+            if (e0 + 4 <= T) {
+                q[j] = *reinterpret_cast<const u32x4*>(row + e0);
+            } else {
+                q[j].x = e0 < T ? row[e0] : 0u;
+                q[j].y = e0 + 1 < T ? row[e0 + 1] : 0u;
+                q[j].z = e0 + 2 < T ? row[e0 + 2] : 0u;
+                q[j].w = 0u;  // e0 + 3 >= T here
             }
-        } else {
-#pragma unroll
-            for (int j = 0; j < PER; ++j) x[j] = (beg + j < T) ? row[beg + j] : 0u;
         }
-        uint32_t s = 0;
 #pragma unroll
-        for (int j = 0; j < PER; ++j) s += x[j];
-        const uint32_t wex = wave_excl_scan(s, lane);
-        if (lane == kWave - 1) wsum[wave] = wex + s;
+        for (int j = 0; j < Q; ++j) {
+            const uint32_t sum = q[j].x + q[j].y + q[j].z + q[j].w;
+            const uint32_t ex = wave_excl_scan(sum, lane);
+            pre[j] = run + ex;
+            run += (uint32_t)__builtin_amdgcn_readlane((int)(ex + sum), kWave - 1);
+        }
+        if (lane == 0) wsum[wave] = run;
         __syncthreads();
         uint32_t wpre = 0, tot = 0;
 #pragma unroll
@@ -251,24 +263,21 @@ __global__ __launch_bounds__(THREADS) void k_tile_scan(
             wpre += (w < (int)wave) ? wsum[w] : 0u;
             tot += wsum[w];
         }
-        uint32_t acc = carry + wpre + wex;
 #pragma unroll
-        for (int j = 0; j < PER; ++j) {
-            const uint32_t e = x[j];
-            x[j] = acc;
-            acc += e;
-        }
-        if (beg + PER <= T) {
-#pragma unroll
-            for (int j = 0; j < PER / 4; ++j) {
-                u32x4 q;
-                q.x = x[4 * j]; q.y = x[4 * j + 1]; q.z = x[4 * j + 2]; q.w = x[4 * j + 3];
-                reinterpret_cast<u32x4*>(row + beg)[j] = q;
+        for (int j = 0; j < Q; ++j) {
+            const uint32_t e0 = seg + j * (kWave * 4) + lane * 4;
+            u32x4 o;
+            o.x = carry + wpre + pre[j];
+            o.y = o.x + q[j].x;
+            o.z = o.y + q[j].y;
+            o.w = o.z + q[j].z;
+            if (e0 + 4 <= T) {
+                *reinterpret_cast<u32x4*>(row + e0) = o;
+            } else {
+                if (e0 < T) row[e0] = o.x;
+                if (e0 + 1 < T) row[e0 + 1] = o.y;
+                if (e0 + 2 < T) row[e0 + 2] = o.z;
             }
-        } else {
-#pragma unroll
-            for (int j = 0; j < PER; ++j)
-                if (beg + j < T) row[beg + j] = x[j];
         }
         carry += tot;
         __syncthreads();
