@@ -473,8 +473,8 @@ __global__ __launch_bounds__(NW* kWave, MINW) void k_partition(
     }
 }
 
-// d_counts[c] = 0 for n == 0 (a fill kernel rather than a memset, so callers
-// may capture the call in a graph).
+// d_counts[c] = 0 for n == 0, and other small fills (a kernel rather than a
+// memset: the graphs this library captures hold kernel nodes only).
 template <int = 0>
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, uint32_t cnt) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x)
