@@ -205,3 +205,21 @@ def test_general_discriminators_invalid(ctx):
         ctx.partition_soa_select([a, b16], [o, torch.empty_like(b16)], md.Mod(4), 0, 1, 0)
     with pytest.raises(md.MdiscrError):
         ctx.partition_soa_select([a, b8], [o, torch.empty_like(b8)], md.Mod(4), 0, 1, 2)
+
+
+def test_filter_soa_four_byte_records(ctx):
+    """The filter over records of contiguous 4-byte columns (one staging pass)."""
+    import torch
+    import mdiscriminator_amd as md
+    for n in (1, 4097, 100003, 1 << 20):
+        key = filter_input(n, 90 + n % 5)
+        cols = [np.arange(n, dtype=np.uint32), key, W.normal_f32_bits(n, 91)]
+        d_in = [dev(c) for c in cols]
+        d_out = [torch.empty_like(t) for t in d_in]
+        d_cnt = torch.empty(2, dtype=torch.int32, device="cuda:0")
+        ctx.filter_soa(d_in, d_out, md.Pred.bits(3, 1), key_col=1, d_counts=d_cnt)
+        torch.cuda.synchronize()
+        ref, ref_cnt = oracle.filter_soa(cols, oracle.PRED_BITS, 3, 1, key_col=1)
+        assert np.array_equal(d_cnt.cpu().numpy().view(np.uint32), ref_cnt)
+        for j, (o, r) in enumerate(zip(d_out, ref)):
+            assert first_diff(host(o, cols[j]), r) < 0, f"n={n} column {j}"

@@ -167,3 +167,33 @@ def test_class_array_key(ctx):
         wide = W.splitmix64(n, m + 2)
         check_soa(ctx, [pay, cls], md.Mod(m), key_col=1, label=f"class key m={m}")
         check_soa(ctx, [cls, wide, pay], md.Mod(m), key_col=0, label=f"class key wide m={m}")
+
+
+@pytest.mark.parametrize("ncols,key_col", [(2, 0), (2, 1), (3, 2), (4, 0), (4, 3)])
+def test_four_byte_records_every_shape(ctx, ncols, key_col):
+    """Records of 2..4 contiguous 4-byte columns (one staging pass for all
+    columns), key in any column, tail tiles and large m included."""
+    import mdiscriminator_amd as md
+    for n, m in ((1, 4), (4095, 7), (4097, 16), (100003, 64), (1 << 20, 32)):
+        cols = [W.uniform_u31(n, 500 + 10 * j + n % 7) for j in range(ncols)]
+        cols[(key_col + 1) % ncols] = np.arange(n, dtype=np.uint32)   # an index column
+        print(f"w4 ncols={ncols} key={key_col} n={n} m={m}", flush=True)
+        check_soa(ctx, cols, md.Mod(m), key_col=key_col, label=f"w4 {ncols}/{key_col} n={n}")
+
+
+def test_four_byte_records_unaligned_key(ctx):
+    """A 4-byte key column that is not 16-byte aligned takes the scalar count path."""
+    import torch
+    import mdiscriminator_amd as md
+    n, m = 70001, 16
+    key = W.uniform_u31(n, 61)
+    pay = W.normal_f32_bits(n, 62)
+    big = dev(np.concatenate([np.zeros(1, dtype=np.uint32), key]))
+    d_key = big[1:]
+    d_pay = dev(pay)
+    outs = [torch.empty(n, dtype=torch.int32, device="cuda:0"), torch.empty_like(d_pay)]
+    ctx.partition_soa([d_key, d_pay], outs, md.Mask(m))
+    torch.cuda.synchronize()
+    ref, _ = oracle.seq_mdiscr_soa([key, pay], m, kind=oracle.MASK)
+    assert np.array_equal(host(outs[0], key), ref[0])
+    assert np.array_equal(host(outs[1], pay), ref[1])
