@@ -341,11 +341,13 @@ __device__ __forceinline__ void tile_offsets(uint32_t (*cursor)[MC], uint32_t* d
 template <int R, bool FULL>
 __device__ __forceinline__ void load_rounds(const uint32_t* __restrict__ in, uint32_t (&v)[R],
                                             uint32_t wbase, uint32_t n, uint32_t lane) {
+    // One per-lane pointer; round r is the constant offset r * 256 bytes, so
+    // the loads share one address register pair.
+    const uint32_t* __restrict__ lp = in + wbase + lane;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        const uint32_t idx = wbase + r * kWave + lane;
-        if (FULL) v[r] = ld_stream(in + idx);
-        else v[r] = idx < n ? ld_stream(in + idx) : 0u;
+        if (FULL) v[r] = ld_stream(lp + r * kWave);
+        else v[r] = wbase + r * kWave + lane < n ? ld_stream(lp + r * kWave) : 0u;
     }
 }
 

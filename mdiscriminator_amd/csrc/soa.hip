@@ -110,18 +110,15 @@ mdiscr_status run_staged(mdiscr_ctx* ctx, const SoaCols& cols, uint32_t n, const
         case ClassKind::Key:
             if (cols.bytes[cols.key] == 1)
                 return dispatch_discr<ST>(
-                    ctx, cols, KeySrc<uint8_t>{static_cast<const uint8_t*>(cols.in[cols.key]),
-                                               cols.stride[cols.key]},
+                    ctx, cols, KeySrc<uint8_t>{static_cast<const uint8_t*>(cols.in[cols.key]), 1u},
                     n, cs.d, d_counts, s, w);
             return dispatch_discr<ST>(
-                ctx, cols, KeySrc<uint32_t>{static_cast<const uint32_t*>(cols.in[cols.key]),
-                                            cols.stride[cols.key]},
+                ctx, cols, KeySrc<uint32_t>{static_cast<const uint32_t*>(cols.in[cols.key]), 1u},
                 n, cs.d, d_counts, s, w);
         case ClassKind::Pred:
             return launch_soa<1, ST>(
                 ctx, cols,
-                KeySrc<uint32_t>{static_cast<const uint32_t*>(cols.in[cols.key]),
-                                 cols.stride[cols.key]},
+                KeySrc<uint32_t>{static_cast<const uint32_t*>(cols.in[cols.key]), 1u},
                 n, DiscrPred{cs.pw[0], cs.pw[1], cs.pw[2], cs.pw[3]}, 2u, d_counts, s, w);
         case ClassKind::Select:
             return dispatch_discr<ST>(
@@ -161,16 +158,15 @@ mdiscr_status run_soa(mdiscr_ctx* ctx, SoaCols& cols, uint64_t n, const ClassSpe
                 : run_staged<uint32_t>(ctx, cols, (uint32_t)n, cs, d_counts, s, w);
 }
 
-// Input and output extents must not overlap, column against column.
+// Input and output columns must not overlap, nor output columns each other.
 mdiscr_status check_overlaps(mdiscr_ctx* ctx, const SoaCols& cols, uint64_t n) {
     for (uint32_t j = 0; j < cols.ncols; ++j) {
-        const uint64_t ej = n ? ((n - 1) * cols.stride[j] + 1) * cols.bytes[j] : 0;
+        const uint64_t ej = n * cols.bytes[j];
         for (uint32_t i = 0; i < cols.ncols; ++i) {
-            const uint64_t ei = n ? ((n - 1) * cols.stride[i] + 1) * cols.bytes[i] : 0;
+            const uint64_t ei = n * cols.bytes[i];
             if (extents_overlap(cols.out[j], ej, cols.in[i], ei))
                 return ctx_fail(ctx, MDISCR_EINVAL, "output column %u overlaps an input", j);
-            if (i != j && cols.stride[i] == 1 && cols.stride[j] == 1 &&
-                extents_overlap(cols.out[j], ej, cols.out[i], ei))
+            if (i != j && extents_overlap(cols.out[j], ej, cols.out[i], ei))
                 return ctx_fail(ctx, MDISCR_EINVAL, "output columns %u and %u overlap", j, i);
         }
     }
@@ -203,7 +199,6 @@ mdiscr_status build_cols(mdiscr_ctx* ctx, uint32_t ncols, const void* const* d_c
         cols.in[j] = d_cols[j];
         cols.out[j] = d_out_cols[j];
         cols.bytes[j] = b;
-        cols.stride[j] = 1;
     }
     return check_overlaps(ctx, cols, n);
 }

@@ -3,8 +3,9 @@
 // Replaces the reference's three-column tuple path (MdiscrHost_tuples.cu.h:
 // 262-429, MdiscrKernels_tuples.cu.h:4-236): the class comes from one key
 // column (ModTuple<N>::apply, MdiscrHost_tuples.cu.h:44-45) and every column
-// moves to the same destination, as raw bits.  Columns carry an element
-// stride (the count pass over array elements reads word 0 at stride LEN).  A
+// moves to the same destination, as raw bits.  A key source carries an
+// element stride (the count pass over array elements reads word 0 at stride
+// LEN); the partition's columns are contiguous.  A
 // 1-byte key column lets a caller-computed class array be the discriminator;
 // a select source reads the class from one of two columns, chosen per element
 // by a third (the tuple discriminator's z ? x % N : y % N form).
@@ -28,7 +29,6 @@ struct SoaCols {
     const void* in[kMaxCols];
     void* out[kMaxCols];
     uint32_t bytes[kMaxCols];
-    uint32_t stride[kMaxCols];  // element stride, in elements of that column
     uint32_t ncols;
     uint32_t key;
 };
@@ -136,29 +136,27 @@ __global__ __launch_bounds__(NW* kWave) void k_src_count(
 // Moves one column of the current tile: load at the tile's element indices,
 // stage at each element's LDS slot, write slot s to out[dest[s]].
 template <typename T, int NW, int R>
-__device__ __forceinline__ void soa_move_column(const void* __restrict__ src_v, uint32_t sstride,
-                                                void* __restrict__ dst_v, uint32_t dstride,
-                                                uint32_t wbase, uint32_t n, bool full,
-                                                const uint32_t (&slot)[R],
+__device__ __forceinline__ void soa_move_column(const void* __restrict__ src_v,
+                                                void* __restrict__ dst_v, uint32_t wbase,
+                                                uint32_t n, bool full, const uint32_t (&slot)[R],
                                                 const uint32_t (&dest)[R], uint32_t valid,
                                                 void* stage_v, uint32_t lane) {
     constexpr int THREADS = NW * kWave;
-    const T* __restrict__ src = static_cast<const T*>(src_v);
+    // one per-lane pointer; round r is a constant offset
+    const T* __restrict__ lp = static_cast<const T*>(src_v) + wbase + lane;
     T* __restrict__ dst = static_cast<T*>(dst_v);
     T* stage = static_cast<T*>(stage_v);
     T x[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const uint32_t idx = wbase + r * kWave + lane;
-        x[r] = (full || idx < n) ? ld_nt(src + (size_t)idx * sstride) : T(0);
-    }
+    for (int r = 0; r < R; ++r)
+        x[r] = (full || wbase + r * kWave + lane < n) ? ld_nt(lp + r * kWave) : T(0);
 #pragma unroll
     for (int r = 0; r < R; ++r) stage[slot[r]] = x[r];
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         const uint32_t s = k * THREADS + threadIdx.x;
-        if (s < valid) __builtin_nontemporal_store(stage[s], dst + (size_t)dest[k] * dstride);
+        if (s < valid) __builtin_nontemporal_store(stage[s], dst + dest[k]);
     }
     __syncthreads();
 }
@@ -166,7 +164,7 @@ __device__ __forceinline__ void soa_move_column(const void* __restrict__ src_v, 
 // The key column is already in registers: stage and write it without a reload.
 template <typename KT, int NW, int R>
 __device__ __forceinline__ void soa_write_keys(const uint32_t (&v)[R], void* __restrict__ dst_v,
-                                               uint32_t dstride, const uint32_t (&slot)[R],
+                                               const uint32_t (&slot)[R],
                                                const uint32_t (&dest)[R], uint32_t valid,
                                                void* stage_v) {
     constexpr int THREADS = NW * kWave;
@@ -178,7 +176,7 @@ __device__ __forceinline__ void soa_write_keys(const uint32_t (&v)[R], void* __r
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         const uint32_t s = k * THREADS + threadIdx.x;
-        if (s < valid) __builtin_nontemporal_store(stage[s], dst + (size_t)dest[k] * dstride);
+        if (s < valid) __builtin_nontemporal_store(stage[s], dst + dest[k]);
     }
     __syncthreads();
 }
@@ -216,10 +214,18 @@ __global__ __launch_bounds__(NW* kWave) void k_partition_soa(
         const uint32_t valid = full ? (uint32_t)TILE : n - tbase;
 
         uint32_t v[R];
+        if constexpr (Src::kKeyInRegs) {
+            // contiguous key column: one per-lane pointer, constant round offsets
+            const typename Src::KeyT* __restrict__ kp = src.key + wbase + lane;
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const uint32_t idx = wbase + r * kWave + lane;
-            v[r] = (full || idx < n) ? src(idx) : 0u;
+            for (int r = 0; r < R; ++r)
+                v[r] = (full || wbase + r * kWave + lane < n) ? (uint32_t)ld_nt(kp + r * kWave) : 0u;
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const uint32_t idx = wbase + r * kWave + lane;
+                v[r] = (full || idx < n) ? src(idx) : 0u;
+            }
         }
         for (int c = lane; c < MC; c += kWave) cursor[wave][c] = 0;
         __builtin_amdgcn_wave_barrier();
@@ -256,29 +262,28 @@ __global__ __launch_bounds__(NW* kWave) void k_partition_soa(
         for (uint32_t j = 0; j < cols.ncols; ++j) {
             if constexpr (Src::kKeyInRegs) {
                 if (j == cols.key) {
-                    soa_write_keys<typename Src::KeyT, NW, R>(v, cols.out[j], cols.stride[j], slot,
-                                                              dest, valid, stage);
+                    soa_write_keys<typename Src::KeyT, NW, R>(v, cols.out[j], slot, dest, valid,
+                                                              stage);
                     continue;
                 }
             }
-            const uint32_t ss = cols.stride[j];
             switch (cols.bytes[j]) {
                 case 1:
-                    soa_move_column<uint8_t, NW, R>(cols.in[j], ss, cols.out[j], ss, wbase, n, full,
+                    soa_move_column<uint8_t, NW, R>(cols.in[j], cols.out[j], wbase, n, full,
                                                     slot, dest, valid, stage, lane);
                     break;
                 case 2:
-                    soa_move_column<uint16_t, NW, R>(cols.in[j], ss, cols.out[j], ss, wbase, n,
-                                                     full, slot, dest, valid, stage, lane);
+                    soa_move_column<uint16_t, NW, R>(cols.in[j], cols.out[j], wbase, n, full,
+                                                     slot, dest, valid, stage, lane);
                     break;
                 case 4:
-                    soa_move_column<uint32_t, NW, R>(cols.in[j], ss, cols.out[j], ss, wbase, n,
-                                                     full, slot, dest, valid, stage, lane);
+                    soa_move_column<uint32_t, NW, R>(cols.in[j], cols.out[j], wbase, n, full,
+                                                     slot, dest, valid, stage, lane);
                     break;
                 default:
                     if constexpr (sizeof(ST) >= 8)
-                        soa_move_column<uint64_t, NW, R>(cols.in[j], ss, cols.out[j], ss, wbase,
-                                                         n, full, slot, dest, valid, stage, lane);
+                        soa_move_column<uint64_t, NW, R>(cols.in[j], cols.out[j], wbase, n, full,
+                                                         slot, dest, valid, stage, lane);
                     break;
             }
         }
