@@ -36,6 +36,9 @@
 #ifndef MDISCR_NT_STORE
 #define MDISCR_NT_STORE 1
 #endif
+#ifndef MDISCR_COUNT_PREFETCH
+#define MDISCR_COUNT_PREFETCH 1
+#endif
 #ifndef MDISCR_COUNT_NT_LOAD
 #define MDISCR_COUNT_NT_LOAD 1
 #endif
@@ -148,6 +151,23 @@ __global__ __launch_bounds__(NW* kWave) void k_chunk_count(
     const uint32_t t0 = chunk * CT;
     const uint32_t t1 = min(t0 + (uint32_t)CT, T);
 
+    // 16-byte loads of a full tile (one per-lane pointer, constant offsets).
+    auto vec_load = [&](uint32_t tt, u32x4 (&dst)[R / 4]) {
+        const u32x4* lp = reinterpret_cast<const u32x4*>(in + (size_t)tt * TILE) + threadIdx.x;
+#pragma unroll
+        for (int k = 0; k < R / 4; ++k) {
+#if MDISCR_COUNT_NT_LOAD
+            dst[k] = __builtin_nontemporal_load(lp + k * THREADS);
+#else
+            dst[k] = lp[k * THREADS];
+#endif
+        }
+    };
+    auto tile_full = [&](uint32_t tt) { return (uint64_t)(tt + 1) * TILE <= n; };
+    // With MDISCR_COUNT_PREFETCH the next full tile's loads are issued before
+    // the current tile is counted.
+    u32x4 q[R / 4];
+    if (VEC && MDISCR_COUNT_PREFETCH && t0 < t1 && tile_full(t0)) vec_load(t0, q);
     for (uint32_t t = t0; t < t1; ++t) {
         uint32_t (*h)[MC] = hist[(t - t0) & 1u];
         if (!PACKED) {
@@ -158,16 +178,10 @@ __global__ __launch_bounds__(NW* kWave) void k_chunk_count(
         const uint32_t tbase = t * (uint32_t)TILE;
         const bool full = (uint64_t)tbase + TILE <= n;
         if (VEC && full) {
-            const u32x4* in4 = reinterpret_cast<const u32x4*>(in + tbase);
-            u32x4 q[R / 4];
-#pragma unroll
-            for (int k = 0; k < R / 4; ++k) {
-#if MDISCR_COUNT_NT_LOAD
-                q[k] = __builtin_nontemporal_load(in4 + k * THREADS + threadIdx.x);
-#else
-                q[k] = in4[k * THREADS + threadIdx.x];
-#endif
-            }
+            u32x4 qn[R / 4];
+            const bool pre = MDISCR_COUNT_PREFETCH && t + 1 < t1 && tile_full(t + 1);
+            if (!MDISCR_COUNT_PREFETCH) vec_load(t, q);
+            if (pre) vec_load(t + 1, qn);
 #pragma unroll
             for (int k = 0; k < R / 4; ++k) {
                 const uint32_t e[4] = {q[k].x, q[k].y, q[k].z, q[k].w};
@@ -177,6 +191,10 @@ __global__ __launch_bounds__(NW* kWave) void k_chunk_count(
                     if (PACKED) lc.add(c);
                     else atomicAdd(&h[wave][c], 1u);
                 }
+            }
+            if (pre) {
+#pragma unroll
+                for (int k = 0; k < R / 4; ++k) q[k] = qn[k];
             }
         } else {
             uint32_t v[R];
