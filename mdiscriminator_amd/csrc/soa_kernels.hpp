@@ -41,6 +41,20 @@ __device__ __forceinline__ T ld_nt(const T* p) {
 // ---------------------------------------------------------------------------
 // Class sources: the 32-bit value the discriminator sees for element idx.
 
+// Every class source loads R elements at once, first + r*step (r < R), with
+// elements at or past n reading 0 unless `full`: gather().  Loads of one
+// column are issued back to back.
+
+// Loads R elements of a contiguous column of T through one per-lane pointer.
+template <typename T, int R>
+__device__ __forceinline__ void gather_col(const T* __restrict__ p, uint32_t first, uint32_t step,
+                                           uint32_t n, bool full, uint32_t (&v)[R]) {
+    const T* __restrict__ lp = p + first;
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        v[r] = (full || first + r * step < n) ? (uint32_t)ld_nt(lp + (size_t)r * step) : 0u;
+}
+
 // One key column (4-byte keys or 1-byte classes) with an element stride.  The
 // values in registers are that column's elements, so the partition writes the
 // key column from registers.
@@ -50,14 +64,25 @@ struct KeySrc {
     static constexpr bool kKeyInRegs = true;
     const KT* key;
     uint32_t stride;
-    __device__ __forceinline__ uint32_t operator()(uint32_t idx) const {
-        return (uint32_t)ld_nt(key + (size_t)idx * stride);
+    template <int R>
+    __device__ __forceinline__ void gather(uint32_t first, uint32_t step, uint32_t n, bool full,
+                                           uint32_t (&v)[R]) const {
+        if (stride == 1) {
+            gather_col<KT, R>(key, first, step, n, full, v);
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const uint32_t idx = first + r * step;
+                v[r] = (full || idx < n) ? (uint32_t)ld_nt(key + (size_t)idx * stride) : 0u;
+            }
+        }
     }
 };
 
 // z ? x : y over three contiguous columns of 1- or 4-byte values.  With one
 // discriminator D, z ? D(x) : D(y) = D(z ? x : y), so the select form of the
-// tuple discriminator costs one D per element.
+// tuple discriminator costs one D per element.  Each column's width is a
+// wave-uniform branch taken once per gather, not per element.
 struct SelectSrc {
     using KeyT = uint32_t;
     static constexpr bool kKeyInRegs = false;
@@ -65,15 +90,22 @@ struct SelectSrc {
     const void* a;
     const void* b;
     uint32_t sel4, a4, b4;  // nonzero when that column holds 4-byte values
-    __device__ __forceinline__ static uint32_t word(const void* p, uint32_t w4, uint32_t idx) {
-        return w4 ? ld_nt(static_cast<const uint32_t*>(p) + idx)
-                  : (uint32_t)ld_nt(static_cast<const uint8_t*>(p) + idx);
+    template <int R>
+    __device__ __forceinline__ static void col(const void* p, uint32_t w4, uint32_t first,
+                                               uint32_t step, uint32_t n, bool full,
+                                               uint32_t (&v)[R]) {
+        if (w4) gather_col<uint32_t, R>(static_cast<const uint32_t*>(p), first, step, n, full, v);
+        else gather_col<uint8_t, R>(static_cast<const uint8_t*>(p), first, step, n, full, v);
     }
-    __device__ __forceinline__ uint32_t operator()(uint32_t idx) const {
-        const uint32_t z = word(sel, sel4, idx);
-        const uint32_t x = word(a, a4, idx);
-        const uint32_t y = word(b, b4, idx);
-        return z ? x : y;
+    template <int R>
+    __device__ __forceinline__ void gather(uint32_t first, uint32_t step, uint32_t n, bool full,
+                                           uint32_t (&v)[R]) const {
+        uint32_t z[R], y[R];
+        col<R>(sel, sel4, first, step, n, full, z);
+        col<R>(a, a4, first, step, n, full, v);
+        col<R>(b, b4, first, step, n, full, y);
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = z[r] ? v[r] : y[r];
     }
 };
 
@@ -104,11 +136,7 @@ __global__ __launch_bounds__(NW* kWave) void k_src_count(
         const uint32_t tbase = t * (uint32_t)TILE;
         const bool full = (uint64_t)tbase + TILE <= n;
         uint32_t v[R];
-#pragma unroll
-        for (int k = 0; k < R; ++k) {
-            const uint32_t idx = tbase + k * THREADS + threadIdx.x;
-            v[k] = (full || idx < n) ? src(idx) : 0u;
-        }
+        src.template gather<R>(tbase + threadIdx.x, THREADS, n, full, v);
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             const uint32_t idx = tbase + k * THREADS + threadIdx.x;
@@ -214,19 +242,10 @@ __global__ __launch_bounds__(NW* kWave) void k_partition_soa(
         const uint32_t valid = full ? (uint32_t)TILE : n - tbase;
 
         uint32_t v[R];
-        if constexpr (Src::kKeyInRegs) {
-            // contiguous key column: one per-lane pointer, constant round offsets
-            const typename Src::KeyT* __restrict__ kp = src.key + wbase + lane;
-#pragma unroll
-            for (int r = 0; r < R; ++r)
-                v[r] = (full || wbase + r * kWave + lane < n) ? (uint32_t)ld_nt(kp + r * kWave) : 0u;
-        } else {
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const uint32_t idx = wbase + r * kWave + lane;
-                v[r] = (full || idx < n) ? src(idx) : 0u;
-            }
-        }
+        if constexpr (Src::kKeyInRegs)  // the partition's key column is contiguous
+            gather_col<typename Src::KeyT, R>(src.key, wbase + lane, kWave, n, full, v);
+        else
+            src.template gather<R>(wbase + lane, kWave, n, full, v);
         for (int c = lane; c < MC; c += kWave) cursor[wave][c] = 0;
         __builtin_amdgcn_wave_barrier();
 
