@@ -223,3 +223,41 @@ def test_filter_soa_four_byte_records(ctx):
         assert np.array_equal(d_cnt.cpu().numpy().view(np.uint32), ref_cnt)
         for j, (o, r) in enumerate(zip(d_out, ref)):
             assert first_diff(host(o, cols[j]), r) < 0, f"n={n} column {j}"
+
+
+def test_filter_full_size_2p28(ctx):
+    """parFilter over the plain configuration's 2^28 keys, against the oracle."""
+    import torch
+    import mdiscriminator_amd as md
+    n = 1 << 28
+    d_in = W.uniform_u31_torch(n, 0x5EED0000 + 2, "cuda:0")
+    x = d_in.cpu().numpy().view(np.uint32)
+    d_out = torch.empty_like(d_in)
+    d_cnt = torch.empty(2, dtype=torch.int32, device="cuda:0")
+    print("filter 2^28", flush=True)
+    ctx.filter_u32(d_in, d_out, md.Pred.lt(1 << 30), d_counts=d_cnt)
+    torch.cuda.synchronize()
+    ref, _, ref_cnt = oracle.par_filter(x, oracle.PRED_LT, 1 << 30)
+    assert np.array_equal(d_cnt.cpu().numpy().view(np.uint32), ref_cnt)
+    assert first_diff(d_out.cpu().numpy().view(np.uint32), ref) < 0
+
+
+def test_select_full_size_2p26(ctx):
+    """The select discriminator over 2^26 (uint, uchar, bool) records."""
+    import torch
+    import mdiscriminator_amd as md
+    n = 1 << 26
+    x = W.uniform_u31(n, 31)
+    y = (W.uniform_u31(n, 32) & 0xFF).astype(np.uint8)
+    z = (W.uniform_u31(n, 33) & 1).astype(np.uint8)
+    cols = [x, y, z]
+    d_in = [dev(c) for c in cols]
+    d_out = [torch.empty_like(t) for t in d_in]
+    d_cnt = torch.empty(8, dtype=torch.int32, device="cuda:0")
+    print("select 2^26", flush=True)
+    ctx.partition_soa_select(d_in, d_out, md.Mask(8), 2, 0, 1, d_counts=d_cnt)
+    torch.cuda.synchronize()
+    ref, ref_cnt = oracle.seq_mdiscr_soa_select(cols, 8, 2, 0, 1, kind=oracle.MASK)
+    assert np.array_equal(d_cnt.cpu().numpy().view(np.uint32), ref_cnt)
+    for j, (o, r) in enumerate(zip(d_out, ref)):
+        assert first_diff(host(o, cols[j]), r) < 0, f"column {j}"

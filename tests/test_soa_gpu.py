@@ -197,3 +197,20 @@ def test_four_byte_records_unaligned_key(ctx):
     ref, _ = oracle.seq_mdiscr_soa([key, pay], m, kind=oracle.MASK)
     assert np.array_equal(host(outs[0], key), ref[0])
     assert np.array_equal(host(outs[1], pay), ref[1])
+
+
+def test_array_elements_full_size(ctx):
+    """2^26 array elements of 4 words (1 GiB), m=16, against the oracle."""
+    import torch
+    import mdiscriminator_amd as md
+    n, length, m = 1 << 26, 4, 16
+    d_in = W.uniform_u31_torch(n * length, 77, "cuda:0")
+    x = d_in.cpu().numpy().view(np.uint32)
+    d_out = torch.empty_like(d_in)
+    d_cnt = torch.empty(m, dtype=torch.int32, device="cuda:0")
+    print("arrays 2^26 x 4", flush=True)
+    ctx.partition_arrays(d_in, d_out, length, md.Mask(m), d_counts=d_cnt)
+    torch.cuda.synchronize()
+    ref, ref_cnt = oracle.seq_mdiscr_arrays(x, length, m, kind=oracle.MASK)
+    assert np.array_equal(d_cnt.cpu().numpy().view(np.uint32), ref_cnt)
+    assert first_diff(d_out.cpu().numpy().view(np.uint32), ref) < 0
