@@ -199,3 +199,24 @@ def test_full_size_10k_lognormal(ctx):
     ref, ref_cnt = oracle.seq_mdiscr_segm(x, off, 4)
     assert first_diff(from_dev(d_out), ref) < 0
     assert np.array_equal(from_dev(d_cnt).reshape(10000, 4), ref_cnt)
+
+
+def test_full_size_10k_lognormal_flags(ctx):
+    """The same 10,000 lognormal segments given as the reference's start-flag
+    array (segment_sizes, MdiscrSegmTest.cu:52-53); out and new_sizes against
+    the oracle's restatement of the flag form."""
+    import torch
+    import mdiscriminator_amd as md
+    L = W.lognormal_lengths(10000, W.SEED_BASE + 4)
+    n = int(L.sum())
+    f = W.flags_from_lengths(L, n)
+    print(f"10k lognormal flags: n={n}", flush=True)
+    d_in = W.uniform_u31_torch(n, W.SEED_BASE + 5, "cuda:0")
+    d_out = torch.empty_like(d_in)
+    d_new = torch.empty_like(d_in)
+    ctx.segm_flags(md.Mod4, n, d_in, torch.from_numpy(f).to("cuda:0"), d_out, d_new)
+    torch.cuda.synchronize()
+    x = from_dev(d_in)
+    ref, ref_new = oracle.segm_flags(x, f, 4)
+    assert first_diff(from_dev(d_out), ref) < 0
+    assert first_diff(d_new.cpu().numpy(), ref_new) < 0
