@@ -14,15 +14,20 @@ using namespace mdiscr;
 
 namespace {
 
-// Tile geometry by staging width and class bits.  8-byte staging (a column
-// wider than 4 bytes): 8 waves x 8 rounds = 4096 elements.  4-byte staging:
-// the same 4096-element tile up to 32 classes, and 16 waves x 12 rounds =
-// 12288 elements for 64 classes, where longer class runs pay for the bigger
-// workgroup (measured on 2^26 three-column records: 8 rounds 0.416 ms,
-// 12 rounds 0.403 ms, 16 rounds 0.426 ms).
+// Tile geometry by staging width and class bits (measured on 2^26
+// three-column records, interleaved A/B).  8-byte staging (a column wider
+// than 4 bytes): 8 waves x 8 rounds = 4096 elements.  4-byte staging: 8 waves
+// x 8 rounds up to 4 classes; 8 waves x 12 rounds = 6144 elements for 8..32
+// classes (m=16: -1.6%, m=32: -2.1%, select at m=8: -11%; m=4 is 2.9% faster
+// at 8 rounds); 16 waves x 12 rounds = 12288 elements for 64 classes (8
+// rounds 0.416 ms, 12 rounds 0.403 ms, 16 rounds 0.426 ms).  16 rounds at 8
+// waves loses 20-30% (occupancy).
 constexpr int kCT = 1;    // tiles per partition workgroup
 constexpr int kCCT = 8;   // tiles per count workgroup
 
+#ifndef MDISCR_SOA_R
+#define MDISCR_SOA_R 12
+#endif
 #ifndef MDISCR_SOA64_R
 #define MDISCR_SOA64_R 12
 #endif
@@ -30,13 +35,21 @@ template <typename ST, int NBITS>
 struct SoaGeom {
     static constexpr bool kLarge = sizeof(ST) == 4 && NBITS >= 6;
     static constexpr int NW = kLarge ? 16 : 8;
-    static constexpr int R = kLarge ? MDISCR_SOA64_R : 8;
+    static constexpr int R =
+        kLarge ? MDISCR_SOA64_R : (sizeof(ST) == 4 && NBITS >= 3 ? MDISCR_SOA_R : 8);
     static constexpr uint32_t TILE = NW * kWave * R;
 };
 
+// The tile launch_soa uses for these columns and classes (the workspace size).
 uint32_t soa_tile(bool wide, uint32_t nbits) {
-    return nbits >= 6 && !wide ? SoaGeom<uint32_t, 6>::TILE : SoaGeom<uint64_t, 0>::TILE;
+    if (wide) return SoaGeom<uint64_t, 0>::TILE;
+    if (nbits >= 6) return SoaGeom<uint32_t, 6>::TILE;
+    return nbits >= 3 ? SoaGeom<uint32_t, 3>::TILE : SoaGeom<uint32_t, 0>::TILE;
 }
+static_assert(SoaGeom<uint32_t, 5>::TILE == SoaGeom<uint32_t, 3>::TILE &&
+                  SoaGeom<uint32_t, 2>::TILE == SoaGeom<uint32_t, 0>::TILE &&
+                  SoaGeom<uint64_t, 6>::TILE == SoaGeom<uint64_t, 0>::TILE,
+              "soa_tile covers every geometry");
 
 template <int NBITS, typename ST, class Src, class Discr>
 mdiscr_status launch_soa(mdiscr_ctx* ctx, const SoaCols& cols, Src src, uint32_t n, Discr dis,
