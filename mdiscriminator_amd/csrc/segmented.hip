@@ -27,6 +27,7 @@ namespace {
 
 constexpr int kPlanThreads = 1024;
 constexpr uint32_t kExpandGrid = 1024;  // workgroups writing the T items of long segments
+constexpr uint32_t kExpandInPlan = 256;  // up to this many segments the planner expands them
 
 struct SegConfig {
     int nw, r;
@@ -47,6 +48,8 @@ constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
 constexpr uint32_t kMinTile = 1024;  // smallest NW*64*R among the candidates
 
 struct SegArgs {
+    const SegDesc* staged = nullptr;  // batch descriptors in mapped pinned memory
+    hipEvent_t staged_read = nullptr;  // recorded once the planner has read them
     const SegDesc* desc;
     SegItem* items;
     SegLong* longs;
@@ -84,10 +87,16 @@ inline uint32_t persistent_grid(int per_cu, const SegArgs& a) {
 
 template <int NW, int R, int NBITS, bool MASK, bool STAGED>
 void launch_cfg(const SegArgs& a, hipStream_t s) {
+    const bool expand_here = a.nseg <= kExpandInPlan;
     hipLaunchKernelGGL((k_seg_plan<kPlanThreads, NW, R>), dim3(1), dim3(kPlanThreads), 0, s,
-                       a.desc, a.nseg, a.items, a.longs, a.plan, a.counts, a.ld, a.mrows);
-    const uint32_t eg = a.nseg < kExpandGrid ? (a.nseg ? a.nseg : 1u) : kExpandGrid;
-    hipLaunchKernelGGL((k_seg_expand<256>), dim3(eg), dim3(256), 0, s, a.longs, a.plan, a.items);
+                       a.staged, const_cast<SegDesc*>(a.desc), a.nseg, a.items, a.longs, a.plan,
+                       a.counts, a.ld, a.mrows, expand_here ? 1u : 0u);
+    if (a.staged_read) (void)hipEventRecord(a.staged_read, s);
+    if (!expand_here) {
+        const uint32_t eg = a.nseg < kExpandGrid ? (a.nseg ? a.nseg : 1u) : kExpandGrid;
+        hipLaunchKernelGGL((k_seg_expand<256>), dim3(eg), dim3(256), 0, s, a.longs, a.plan,
+                           a.items);
+    }
     static const int count_per_cu = resident_per_cu(k_seg_count<NW, R, NBITS, MASK>, NW * kWave);
     static const int part_per_cu =
         resident_per_cu(k_seg_partition<NW, R, NBITS, MASK, STAGED>, NW * kWave);
@@ -629,15 +638,13 @@ mdiscr_status mdiscr_partition_batch(mdiscr_ctx* ctx, const mdiscr_request* reqs
     SegWorkspace w;
     st = seg_workspace(ctx, total, nreq, mmax, w);
     if (st) return st;
-    hipLaunchKernelGGL(k_seg_desc_copy, dim3((nreq + 255) / 256), dim3(256), 0, s,
-                       static_cast<const SegDesc*>(ss->host_desc_dev), nreq, w.desc);
-    st = ctx_hip(ctx, hipEventRecord(ss->staged_evt, s), "hipEventRecord");
-    if (st) return st;
     const uint32_t nbits = class_bits(mmax);
     int cfg = 0;
     auto it = ss->tuned.find(bucket_key(all_pow2, nbits, total, nreq));
     if (it != ss->tuned.end() && config_available(all_pow2, nbits, it->second)) cfg = it->second;
     SegArgs a;
+    a.staged = static_cast<const SegDesc*>(ss->host_desc_dev);  // the planner copies them
+    a.staged_read = ss->staged_evt;
     a.desc = w.desc;
     a.items = w.items;
     a.longs = w.longs;

@@ -86,13 +86,6 @@ __global__ void k_seg_desc_from_offsets(const uint64_t* __restrict__ off, uint32
     desc[s] = d;
 }
 
-// Descriptors of a batch, read straight from mapped pinned host memory.
-__global__ void k_seg_desc_copy(const SegDesc* __restrict__ host, uint32_t nseg,
-                                SegDesc* __restrict__ desc) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < nseg) desc[s] = host[s];
-}
-
 // ---------------------------------------------------------------------------
 // Planning: one workgroup of THREADS walks the segments in passes, with
 // block scans for the item and row counts.
@@ -115,12 +108,13 @@ __device__ __forceinline__ uint32_t block_excl_sum(uint32_t v, uint32_t* wsum, u
 }
 
 template <int THREADS, int NW, int R>
-__global__ __launch_bounds__(THREADS) void k_seg_plan(const SegDesc* __restrict__ desc,
-                                                      uint32_t nseg, SegItem* __restrict__ items,
+__global__ __launch_bounds__(THREADS) void k_seg_plan(const SegDesc* __restrict__ staged,
+                                                      SegDesc* __restrict__ desc, uint32_t nseg,
+                                                      SegItem* __restrict__ items,
                                                       SegLong* __restrict__ longs,
                                                       SegPlan* __restrict__ plan,
                                                       uint32_t* __restrict__ counts, uint32_t ld,
-                                                      uint32_t mrows) {
+                                                      uint32_t mrows, uint32_t expand_here) {
     // Segments are grouped by index, NW per group; a group's short segments
     // (len <= 64*R) form one W item whose bit w marks segment first+w.  Each
     // thread owns SPT consecutive segments (whole groups) per pass.  A
@@ -133,6 +127,12 @@ __global__ __launch_bounds__(THREADS) void k_seg_plan(const SegDesc* __restrict_
     constexpr uint32_t SPT = 16;
     static_assert(SPT % NW == 0, "whole groups per thread");
     __shared__ uint32_t wsum[THREADS / kWave];
+    // A batch's descriptors arrive in mapped pinned memory: copy them to the
+    // device table the later kernels read.
+    if (staged) {
+        for (uint32_t i = threadIdx.x; i < nseg; i += THREADS) desc[i] = staged[i];
+        __syncthreads();
+    }
     uint32_t item_carry = 0, row_carry = 0, long_carry = 0;
     for (uint32_t base = 0; base < nseg; base += THREADS * SPT) {
         const uint32_t s0 = base + threadIdx.x * SPT;
@@ -190,6 +190,17 @@ __global__ __launch_bounds__(THREADS) void k_seg_plan(const SegDesc* __restrict_
     if (threadIdx.x == 0) *plan = SegPlan{item_carry, row_carry, long_carry};
     // A zero after the last row: the scan's exclusive prefix there is the total.
     for (uint32_t c = threadIdx.x; c < mrows; c += THREADS) counts[(size_t)c * ld + row_carry] = 0u;
+    // Few segments (a serving batch): write the multi-tile segments' T items
+    // here, the whole block on one segment at a time, instead of launching
+    // k_seg_expand.
+    if (expand_here) {
+        __syncthreads();
+        for (uint32_t k = 0; k < long_carry; ++k) {
+            const SegLong L = longs[k];
+            for (uint32_t j = threadIdx.x; j < L.tiles; j += THREADS)
+                items[L.item + j] = SegItem{L.seg, j, L.tiles, L.row + j};
+        }
+    }
 }
 
 // The T items of the multi-tile segments: one workgroup per SegLong record.
