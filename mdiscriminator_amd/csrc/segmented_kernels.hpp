@@ -244,11 +244,12 @@ __global__ __launch_bounds__(NW* kWave) void k_seg_count(const SegDesc* __restri
         }
         const uint32_t tstart = it.j * TILE;
         const bool full = tstart + TILE <= d.len;
+        const uint32_t* __restrict__ lp = d.in + tstart + threadIdx.x;  // constant offsets below
         uint32_t v[R];
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             const uint32_t idx = tstart + k * THREADS + threadIdx.x;
-            v[k] = (full || idx < d.len) ? __builtin_nontemporal_load(d.in + idx) : 0u;
+            v[k] = (full || idx < d.len) ? __builtin_nontemporal_load(lp + k * THREADS) : 0u;
         }
         LaneCounts<NBITS> lc;
 #pragma unroll
