@@ -64,14 +64,26 @@ mdiscr_status ctx_workspace(mdiscr_ctx* ctx, size_t bytes, void** out) {
 void timing_begin(mdiscr_ctx* ctx, hipStream_t s, TimedLaunch& t) {
     t.on = ctx->timing;
     if (!t.on) return;
-    hipEventCreate(&t.a);
-    hipEventCreate(&t.b);
-    hipEventRecord(t.a, s);
+    t.a = t.b = nullptr;
+    // Timing is a diagnostic: a launch whose events cannot be made or
+    // recorded runs untimed rather than failing.
+    if (hipEventCreate(&t.a) != hipSuccess || hipEventCreate(&t.b) != hipSuccess ||
+        hipEventRecord(t.a, s) != hipSuccess) {
+        (void)hipGetLastError();
+        if (t.a) (void)hipEventDestroy(t.a);
+        if (t.b) (void)hipEventDestroy(t.b);
+        t.on = false;
+    }
 }
 
 void timing_end(mdiscr_ctx* ctx, hipStream_t s, TimedLaunch& t, const char* name) {
     if (!t.on) return;
-    hipEventRecord(t.b, s);
+    if (hipEventRecord(t.b, s) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+        return;
+    }
     t.name = name;
     ctx->pending.push_back(t);
 }
@@ -145,18 +157,24 @@ mdiscr_status mdiscr_timing_read(mdiscr_ctx* ctx, const char* name, double* tota
                                  uint64_t* launches) {
     if (!ctx || !name || !total_ms || !launches) return MDISCR_EINVAL;
     // Fold every finished record into the per-name totals, then report one.
-    for (auto& t : ctx->pending) {
-        hipError_t e = hipEventSynchronize(t.b);
-        if (e != hipSuccess) return ctx_hip(ctx, e, "hipEventSynchronize");
+    size_t done = 0;
+    hipError_t e = hipSuccess;
+    for (; done < ctx->pending.size(); ++done) {
+        const TimedLaunch& t = ctx->pending[done];
+        e = hipEventSynchronize(t.b);
+        if (e != hipSuccess) break;
         float ms = 0.f;
-        hipEventElapsedTime(&ms, t.a, t.b);
+        if (hipEventElapsedTime(&ms, t.a, t.b) != hipSuccess) (void)hipGetLastError();
         auto& acc = ctx->timing_acc[t.name];
         acc.first += ms;
         acc.second += 1;
-        hipEventDestroy(t.a);
-        hipEventDestroy(t.b);
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
     }
-    ctx->pending.clear();
+    // Records already folded (and their events destroyed) leave the list even
+    // when a later one fails, so a retry never touches a destroyed event.
+    ctx->pending.erase(ctx->pending.begin(), ctx->pending.begin() + done);
+    if (e != hipSuccess) return ctx_hip(ctx, e, "hipEventSynchronize");
     auto it = ctx->timing_acc.find(name);
     if (it == ctx->timing_acc.end()) {
         *total_ms = 0.0;

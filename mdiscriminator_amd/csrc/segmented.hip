@@ -50,6 +50,7 @@ constexpr uint32_t kMinTile = 1024;  // smallest NW*64*R among the candidates
 struct SegArgs {
     const SegDesc* staged = nullptr;  // batch descriptors in mapped pinned memory
     hipEvent_t staged_read = nullptr;  // recorded once the planner has read them
+    hipError_t* staged_err = nullptr;  // where that record's status goes
     const SegDesc* desc;
     SegItem* items;
     SegLong* longs;
@@ -91,7 +92,7 @@ void launch_cfg(const SegArgs& a, hipStream_t s) {
     hipLaunchKernelGGL((k_seg_plan<kPlanThreads, NW, R>), dim3(1), dim3(kPlanThreads), 0, s,
                        a.staged, const_cast<SegDesc*>(a.desc), a.nseg, a.items, a.longs, a.plan,
                        a.counts, a.ld, a.mrows, expand_here ? 1u : 0u);
-    if (a.staged_read) (void)hipEventRecord(a.staged_read, s);
+    if (a.staged_read) *a.staged_err = hipEventRecord(a.staged_read, s);
     if (!expand_here) {
         const uint32_t eg = a.nseg < kExpandGrid ? (a.nseg ? a.nseg : 1u) : kExpandGrid;
         hipLaunchKernelGGL((k_seg_expand<256>), dim3(eg), dim3(256), 0, s, a.longs, a.plan,
@@ -370,20 +371,21 @@ int pick_config(mdiscr_ctx* ctx, const uint32_t* d_in, uint32_t* d_out, uint64_t
         if (!config_available(mask, nbits, c)) continue;
         float mn = 1e30f;
         for (int rep = 0; rep < 4; ++rep) {
-            (void)hipEventRecord(a, s);
-            mdiscr_status st = run_segmented(ctx, d_in, d_out, n, d_seg_off, nseg, d, nullptr, s, c,
-                                             false);
-            (void)hipEventRecord(b, s);
+            mdiscr_status st = ctx_hip(ctx, hipEventRecord(a, s), "hipEventRecord");
+            if (!st)
+                st = run_segmented(ctx, d_in, d_out, n, d_seg_off, nseg, d, nullptr, s, c, false);
+            if (!st) st = ctx_hip(ctx, hipEventRecord(b, s), "hipEventRecord");
+            if (!st) st = ctx_hip(ctx, hipEventSynchronize(b), "hipEventSynchronize");
+            float ms = 0.f;
+            if (!st) st = ctx_hip(ctx, hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
             if (st) {
+                // A failed trial must not be remembered as the fastest choice.
                 *err = st;
                 ctx->timing = saved_timing;
                 (void)hipEventDestroy(a);
                 (void)hipEventDestroy(b);
                 return 0;
             }
-            (void)hipEventSynchronize(b);
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, a, b);
             if (rep > 0 && ms < mn) mn = ms;
         }
         if (mn < best_ms) {
@@ -644,7 +646,9 @@ mdiscr_status mdiscr_partition_batch(mdiscr_ctx* ctx, const mdiscr_request* reqs
     if (it != ss->tuned.end() && config_available(all_pow2, nbits, it->second)) cfg = it->second;
     SegArgs a;
     a.staged = static_cast<const SegDesc*>(ss->host_desc_dev);  // the planner copies them
+    hipError_t staged_err = hipSuccess;
     a.staged_read = ss->staged_evt;
+    a.staged_err = &staged_err;
     a.desc = w.desc;
     a.items = w.items;
     a.longs = w.longs;
@@ -666,7 +670,12 @@ mdiscr_status mdiscr_partition_batch(mdiscr_ctx* ctx, const mdiscr_request* reqs
     const bool ok = all_pow2 ? launch_any<true>(nbits, cfg, a, s) : launch_any<false>(nbits, cfg, a, s);
     timing_end(ctx, s, tl, "batch");
     if (!ok) return ctx_fail(ctx, MDISCR_EUNSUPPORTED, "batch config");
-    return ctx_hip(ctx, hipGetLastError(), "batch launch");
+    st = ctx_hip(ctx, hipGetLastError(), "batch launch");
+    if (!st) st = ctx_hip(ctx, staged_err, "hipEventRecord");
+    // After a failure the staging event may not cover this batch's planner:
+    // drain the stream so that the next batch can restage safely.
+    if (st) (void)hipStreamSynchronize(s);
+    return st;
 }
 
 }  // extern "C"

@@ -120,16 +120,24 @@ struct mdiscr_server {
         in_use -= cls;
     }
 
-    void retire(Slot& sl) {  // mu held
+    // Retire a slot's batch (mu held): completed, or failed when the device
+    // reported an error for it, so that no waiter blocks on a batch that will
+    // never signal.
+    void retire(Slot& sl, mdiscr_status err) {
         const Clock::time_point now = Clock::now();
         for (size_t i = 0; i < sl.tickets.size(); ++i) {
             auto it = reqs.find(sl.tickets[i]);
             if (it == reqs.end()) continue;
             Request& r = it->second;
+            r.t_done = now;
+            if (err != MDISCR_OK) {
+                r.state = 4;
+                r.err = err;
+                continue;
+            }
             std::copy(sl.h_counts + i * MDISCR_MAX_CLASSES,
                       sl.h_counts + i * MDISCR_MAX_CLASSES + r.d.m, r.h_counts);
             r.state = 3;
-            r.t_done = now;
             latencies.push_back(std::chrono::duration<double>(r.t_done - r.t_submit).count());
             ++completed;
             elems += r.n;
@@ -144,11 +152,14 @@ struct mdiscr_server {
         while (true) {
             bool progressed = false;
             // 1. retire finished batches
-            for (auto& sl : slots)
-                if (sl.busy && hipEventQuery(sl.done) == hipSuccess) {
-                    retire(sl);
-                    progressed = true;
-                }
+            for (auto& sl : slots) {
+                if (!sl.busy) continue;
+                const hipError_t q = hipEventQuery(sl.done);
+                if (q == hipErrorNotReady) continue;
+                if (q != hipSuccess) (void)hipGetLastError();
+                retire(sl, q == hipSuccess ? MDISCR_OK : MDISCR_EDEVICE);
+                progressed = true;
+            }
             if (progressed) cv_done.notify_all();
             // 2. admit in FIFO order while outputs fit the budget
             while (!queued.empty()) {
