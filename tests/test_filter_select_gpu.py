@@ -261,3 +261,36 @@ def test_select_full_size_2p26(ctx):
     assert np.array_equal(d_cnt.cpu().numpy().view(np.uint32), ref_cnt)
     for j, (o, r) in enumerate(zip(d_out, ref)):
         assert first_diff(host(o, cols[j]), r) < 0, f"column {j}"
+
+
+def test_select_misaligned_columns(ctx):
+    """Select columns off their natural word boundaries: a 4-byte key that is
+    not 16-byte aligned and byte columns that are not 4-byte aligned."""
+    import torch
+    import mdiscriminator_amd as md
+    n, m = 70001, 8
+    x = W.uniform_u31(n, 41)
+    y = (W.uniform_u31(n, 42) & 0xFF).astype(np.uint8)
+    z = (W.uniform_u31(n, 43) & 1).astype(np.uint8)
+    cols = [x, y, z]
+
+    def place(a, off):
+        raw = np.zeros(a.nbytes + 16, dtype=np.uint8)
+        raw[off:off + a.nbytes] = a.view(np.uint8)
+        v = torch.from_numpy(raw).to("cuda:0")[off:off + a.nbytes]
+        return v if a.dtype == np.uint8 else v.view(torch.int32)
+
+    for off_x, off_b in ((4, 0), (0, 1), (8, 3)):
+        print(f"select x+{off_x} bytes+{off_b}", flush=True)
+        d_in = [place(x, off_x), place(y, off_b), place(z, off_b)]
+        d_out = [torch.empty(n, dtype=torch.int32, device="cuda:0"),
+                 torch.empty(n, dtype=torch.uint8, device="cuda:0"),
+                 torch.empty(n, dtype=torch.uint8, device="cuda:0")]
+        d_cnt = torch.empty(m, dtype=torch.int32, device="cuda:0")
+        ctx.partition_soa_select(d_in, d_out, md.Mod(m), sel_col=2, a_col=0, b_col=1,
+                                 d_counts=d_cnt)
+        torch.cuda.synchronize()
+        ref, ref_cnt = oracle.seq_mdiscr_soa_select(cols, m, 2, 0, 1)
+        assert np.array_equal(d_cnt.cpu().numpy().view(np.uint32), ref_cnt)
+        for j, (o, r) in enumerate(zip(d_out, ref)):
+            assert first_diff(host(o, cols[j]), r) < 0, f"x+{off_x} b+{off_b} column {j}"

@@ -214,3 +214,53 @@ def test_array_elements_full_size(ctx):
     ref, ref_cnt = oracle.seq_mdiscr_arrays(x, length, m, kind=oracle.MASK)
     assert np.array_equal(d_cnt.cpu().numpy().view(np.uint32), ref_cnt)
     assert first_diff(d_out.cpu().numpy().view(np.uint32), ref) < 0
+
+
+def offset_view(a, off):
+    """A device copy of `a` that starts `off` bytes into a larger buffer."""
+    import torch
+    raw = np.zeros(a.nbytes + 16, dtype=np.uint8)
+    raw[off:off + a.nbytes] = np.ascontiguousarray(a).view(np.uint8)
+    big = torch.from_numpy(raw).to("cuda:0")
+    v = big[off:off + a.nbytes]
+    return v if a.dtype == np.uint8 else v.view(dev(a[:0]).dtype)
+
+
+@pytest.mark.parametrize("m", [1, 2, 3, 8, 33, 64])
+def test_byte_columns_every_m(ctx, m):
+    """1-byte columns: a 4-byte key with byte payloads, and records of byte
+    columns only with a byte class key, at tile and wave boundaries."""
+    import mdiscriminator_amd as md
+    for n in (1, 3, 255, 256, 1000, 4097, 6143, 6144, 6145, 12289, 100003):
+        key = W.uniform_u31(n, 700 + m)
+        cls = (W.uniform_u31(n, 701 + m) % m).astype(np.uint8)
+        b1 = (W.uniform_u31(n, 702 + m) & 0xFF).astype(np.uint8)
+        b2 = (W.uniform_u31(n, 703 + m) & 1).astype(np.uint8)
+        idx = np.arange(n, dtype=np.uint32)
+        print(f"bytes m={m} n={n}", flush=True)
+        check_soa(ctx, [key, b1, idx, b2], md.Mod(m), label=f"u32 key m={m} n={n}")
+        check_soa(ctx, [b1, cls, b2], md.Mod(m), key_col=1, label=f"byte key m={m} n={n}")
+
+
+def test_byte_columns_misaligned(ctx):
+    """Byte columns, and a byte class key, whose input or output does not
+    start on a 4-byte boundary."""
+    import torch
+    import mdiscriminator_amd as md
+    n, m = 50003, 8
+    key = W.uniform_u31(n, 91)
+    cls = (W.uniform_u31(n, 92) % m).astype(np.uint8)
+    b1 = (W.uniform_u31(n, 93) & 0xFF).astype(np.uint8)
+    for off_in, off_out in ((1, 0), (0, 1), (2, 3), (3, 3), (0, 0)):
+        print(f"misaligned in+{off_in} out+{off_out}", flush=True)
+        cols = [key, b1, cls]
+        d_in = [dev(key), offset_view(b1, off_in), offset_view(cls, off_in)]
+        d_out = [torch.empty_like(d_in[0]), offset_view(np.zeros_like(b1), off_out),
+                 offset_view(np.zeros_like(cls), off_out)]
+        for key_col in (0, 2):
+            ctx.partition_soa(d_in, d_out, md.Mod(m), key_col=key_col)
+            torch.cuda.synchronize()
+            ref, _ = oracle.seq_mdiscr_soa(cols, m, key_col=key_col)
+            for j, (o, r) in enumerate(zip(d_out, ref)):
+                assert first_diff(host(o, cols[j]), r) < 0, \
+                    f"in+{off_in} out+{off_out} key {key_col} column {j}"
