@@ -59,7 +59,7 @@ __global__ __launch_bounds__(NW* kWave) void k_partition_rows(
     __shared__ uint32_t dpos[MC];
 
     const uint32_t lane = lane_id();
-    const uint32_t wave = threadIdx.x / kWave;
+    const uint32_t wave = wave_id();
     const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
     const uint32_t tbase = t * (uint32_t)TILE;
     const uint32_t wbase = tbase + wave * (uint32_t)(kWave * R);

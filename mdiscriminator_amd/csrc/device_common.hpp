@@ -56,6 +56,12 @@ __device__ __forceinline__ uint32_t lane_id() {
     return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
 }
 
+// This wave's index in the workgroup, made a scalar (it is wave-uniform), so
+// that wave-relative bases and addresses need no vector registers.
+__device__ __forceinline__ uint32_t wave_id() {
+    return __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+}
+
 // Number of set bits of `mask` in lanes strictly below this lane.
 __device__ __forceinline__ uint32_t mbcnt64(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),

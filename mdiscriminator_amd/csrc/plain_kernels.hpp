@@ -145,7 +145,7 @@ __global__ __launch_bounds__(NW* kWave) void k_chunk_count(
     __shared__ uint32_t hist[2][NW][MC];  // double-buffered across tiles
 
     const uint32_t lane = lane_id();
-    const uint32_t wave = threadIdx.x / kWave;
+    const uint32_t wave = wave_id();
     const uint32_t chunk = xcd_tile(blockIdx.x, gridDim.x);
     const uint32_t T = (n + TILE - 1) / TILE;
     const uint32_t t0 = chunk * CT;
@@ -411,7 +411,7 @@ __global__ __launch_bounds__(NW* kWave, MINW) void k_partition(
     __shared__ uint32_t dpos[MC];        // global destination - LDS slot, per class
 
     const uint32_t lane = lane_id();
-    const uint32_t wave = threadIdx.x / kWave;
+    const uint32_t wave = wave_id();
 #if MDISCR_PLAIN_REVERSE
     const uint32_t chunk = xcd_tile_reversed(blockIdx.x, gridDim.x);
 #else

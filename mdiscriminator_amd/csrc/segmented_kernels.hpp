@@ -230,7 +230,7 @@ __global__ __launch_bounds__(NW* kWave) void k_seg_count(const SegDesc* __restri
     constexpr bool PACKED = LaneCounts<NBITS>::kPacked;
     __shared__ uint32_t hist[NW][MC];
     const uint32_t lane = lane_id();
-    const uint32_t wave = threadIdx.x / kWave;
+    const uint32_t wave = wave_id();
     const uint32_t n_items = plan->n_items;
     for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x) {
         const SegItem it = items[i];
@@ -293,7 +293,7 @@ __global__ __launch_bounds__(NW* kWave) void k_seg_partition(
     __shared__ uint32_t dpos[MC];
 
     const uint32_t lane = lane_id();
-    const uint32_t wave = threadIdx.x / kWave;
+    const uint32_t wave = wave_id();
     const uint32_t n_items = plan->n_items;
     for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x) {
         const SegItem it = items[i];

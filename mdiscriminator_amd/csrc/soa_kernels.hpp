@@ -121,7 +121,7 @@ __global__ __launch_bounds__(NW* kWave) void k_src_count(
     static_assert(R <= 255, "byte counters per lane");
     __shared__ uint32_t hist[2][NW][MC];
     const uint32_t lane = lane_id();
-    const uint32_t wave = threadIdx.x / kWave;
+    const uint32_t wave = wave_id();
     const uint32_t chunk = xcd_tile(blockIdx.x, gridDim.x);
     const uint32_t T = (n + TILE - 1) / TILE;
     const uint32_t t0 = chunk * CT;
@@ -222,7 +222,7 @@ __global__ __launch_bounds__(NW* kWave) void k_partition_soa(
     __shared__ uint32_t dpos[MC];
 
     const uint32_t lane = lane_id();
-    const uint32_t wave = threadIdx.x / kWave;
+    const uint32_t wave = wave_id();
     const uint32_t chunk = xcd_tile(blockIdx.x, gridDim.x);
     const uint32_t T = (n + TILE - 1) / TILE;
     const uint32_t t0 = chunk * CT;
