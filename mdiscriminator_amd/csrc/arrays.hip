@@ -144,7 +144,8 @@ mdiscr_status launch_rows(mdiscr_ctx* ctx, const uint32_t* in, uint32_t* out, ui
     const uint32_t T = (uint32_t)(((uint64_t)n + TILE - 1) / TILE);
     const uint32_t ld = (T + 3u) & ~3u;
     uint32_t* counts = ws;
-    uint32_t* totals = ws + (size_t)m * ld;
+    uint32_t* prefix = ws + (size_t)m * ld;
+    uint32_t* totals = ws + (size_t)2 * m * ld;
     TimedLaunch tl;
     timing_begin(ctx, s, tl);
     hipLaunchKernelGGL((k_src_count<kNW, R, kCCT, NBITS, KeySrc<uint32_t>, Discr>),
@@ -152,12 +153,11 @@ mdiscr_status launch_rows(mdiscr_ctx* ctx, const uint32_t* in, uint32_t* out, ui
                        KeySrc<uint32_t>{in, (uint32_t)LEN}, n, dis, counts, ld, m);
     timing_end(ctx, s, tl, "rows_count");
     timing_begin(ctx, s, tl);
-    hipLaunchKernelGGL((k_tile_scan<1024, 64>), dim3(m), dim3(1024), 0, s, counts, T, ld, totals,
-                       user_counts, (const uint32_t*)nullptr);
+    launch_tile_scan(counts, prefix, T, ld, m, totals, user_counts, nullptr, s);
     timing_end(ctx, s, tl, "rows_scan");
     timing_begin(ctx, s, tl);
     hipLaunchKernelGGL((k_partition_rows<kNW, R, LEN, NBITS, Discr>), dim3(T), dim3(kNW * kWave),
-                       0, s, in, out, n, dis, counts, totals, ld, m);
+                       0, s, in, out, n, dis, prefix, totals, ld, m);
     timing_end(ctx, s, tl, "rows_partition");
     return ctx_hip(ctx, hipGetLastError(), "array partition launch");
 }
@@ -224,7 +224,7 @@ extern "C" mdiscr_status mdiscr_partition_arrays(mdiscr_ctx* ctx, const uint32_t
     const uint64_t T = (n + tile - 1) / tile;
     const uint64_t ld = (T + 3) & ~3ull;
     void* ws = nullptr;
-    st = ctx_workspace(ctx, (ld + 1) * d.m * sizeof(uint32_t), &ws);
+    st = ctx_workspace(ctx, (2 * ld + 1) * d.m * sizeof(uint32_t), &ws);  // counts, prefix, totals
     if (st) return st;
     uint32_t* w = static_cast<uint32_t*>(ws);
     if (d.kind == MDISCR_DISCR_MASK || (d.m & (d.m - 1)) == 0)

@@ -220,8 +220,6 @@ constexpr int kR = MDISCR_PLAIN_R;    // rounds (elements per lane per tile)
 constexpr int kCT = MDISCR_PLAIN_CT;  // tiles per partition workgroup
 constexpr int kCCT = MDISCR_COUNT_CT; // tiles per count workgroup
 constexpr bool kPrefetch = MDISCR_PLAIN_PREFETCH != 0;
-constexpr int kScanThreads = 1024;
-constexpr int kScanPer = 64;
 using PlainGeom = Geom<kNW, kR, kCT>;
 constexpr uint32_t kTile = PlainGeom::kTile;
 
@@ -233,8 +231,9 @@ mdiscr_status launch_plain(mdiscr_ctx* ctx, const uint32_t* in, uint32_t* out, u
     const uint32_t ld = (T + 3u) & ~3u;     // 16-byte aligned class rows
     const uint32_t Kc = (T + kCCT - 1) / kCCT;
     const uint32_t Kp = (T + kCT - 1) / kCT;
-    uint32_t* counts = ws;                  // m * ld
-    uint32_t* totals = ws + (size_t)m * ld;
+    uint32_t* counts = ws;                      // m * ld
+    uint32_t* prefix = ws + (size_t)m * ld;     // m * ld
+    uint32_t* totals = ws + (size_t)2 * m * ld; // m
     const bool vec = ((uintptr_t)in & 15u) == 0;
     TimedLaunch tl;
     timing_begin(ctx, s, tl);
@@ -246,12 +245,11 @@ mdiscr_status launch_plain(mdiscr_ctx* ctx, const uint32_t* in, uint32_t* out, u
                            dim3(kNW * kWave), 0, s, in, n, dis, counts, ld, m);
     timing_end(ctx, s, tl, "chunk_count");
     timing_begin(ctx, s, tl);
-    hipLaunchKernelGGL((k_tile_scan<kScanThreads, kScanPer>), dim3(m), dim3(kScanThreads), 0, s,
-                       counts, T, ld, totals, user_counts);
+    launch_tile_scan(counts, prefix, T, ld, m, totals, user_counts, nullptr, s);
     timing_end(ctx, s, tl, "tile_scan");
     timing_begin(ctx, s, tl);
     hipLaunchKernelGGL((k_partition<kNW, kR, kCT, NBITS, kPrefetch, MDISCR_PLAIN_WB, MDISCR_PLAIN_MINW, Discr>),
-                       dim3(Kp), dim3(kNW * kWave), 0, s, in, out, n, dis, counts, totals, ld, m);
+                       dim3(Kp), dim3(kNW * kWave), 0, s, in, out, n, dis, prefix, totals, ld, m);
     timing_end(ctx, s, tl, "partition");
     return ctx_hip(ctx, hipGetLastError(), "plain partition launch");
 }
@@ -294,7 +292,7 @@ mdiscr_status plain_prologue(mdiscr_ctx* ctx, const uint32_t* d_in, uint32_t* d_
     const uint64_t T = (n + kTile - 1) / kTile;
     const uint64_t ld = (T + 3) & ~3ull;
     void* w = nullptr;
-    st = ctx_workspace(ctx, (ld + 1) * m * sizeof(uint32_t), &w);
+    st = ctx_workspace(ctx, (2 * ld + 1) * m * sizeof(uint32_t), &w);  // counts, prefix, totals
     if (st) return st;
     *ws = static_cast<uint32_t*>(w);
     return MDISCR_OK;

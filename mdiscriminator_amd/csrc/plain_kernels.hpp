@@ -6,8 +6,9 @@
 //
 //   chunk_count reads CT tiles per workgroup once and writes every tile's m
 //               class counts (class-major, counts[c*ld + t]).        4 B/elem
-//   tile_scan   one workgroup per class: exclusive scan of that class's
-//               counts over the T tiles, in place; writes the class total.
+//   tile_scan   exclusive scan of each class's counts over the T tiles into
+//               a prefix array, a few workgroups per class; writes the
+//               class totals.
 //   partition   CT tiles per workgroup in order (one, as launched), carrying
 //               each class's running global offset; reads each tile once,
 //               ranks every element inside its wave with
@@ -228,28 +229,61 @@ __global__ __launch_bounds__(NW* kWave) void k_chunk_count(
 }
 
 // ---------------------------------------------------------------------------
-// tile scan: for class c = blockIdx.x, row = counts + c*ld (ld a multiple of
-// 4, so rows are 16-byte aligned): row[t] <- sum_{t'<t} row[t'] for t < T,
-// totals[c] <- the class total (and user_counts[c] when given).  A pass
-// covers THREADS*PER entries; wave w owns the contiguous segment of 64*PER
-// of them, and its lane l reads and writes entries 256j + 4l .. 256j + 4l + 3
-// of the segment (j < PER/4), so every load and store instruction of a wave
-// touches 1 KiB of consecutive entries.
+// tile scan: prefix[c*ld + t] <- sum_{t' < t} counts[c*ld + t'] for t < T, and
+// the class total to totals[c] (and user_counts[c] when given).  Workgroup
+// (c, k) = (blockIdx.x, blockIdx.y) covers chunk k of class c's row, entries
+// [k*chunk, (k+1)*chunk).  It first sums the counts before its chunk itself
+// (read only, so no workgroup waits on another; a row has few chunks), then
+// scans the chunk in passes of THREADS*PER entries: wave w owns the
+// contiguous segment of 64*PER of them, and lane l reads and writes entries
+// 256j + 4l .. 256j + 4l + 3 of that segment (j < PER/4), so each load and
+// store instruction of a wave touches 1 KiB of consecutive entries.  Rows
+// start 16-byte aligned (ld a multiple of 4) and chunk is a multiple of
+// THREADS*PER.  prefix may be counts itself only with one chunk per row.
 template <int THREADS, int PER>
 __global__ __launch_bounds__(THREADS) void k_tile_scan(
-    uint32_t* __restrict__ counts, uint32_t T, uint32_t ld, uint32_t* __restrict__ totals,
-    uint32_t* __restrict__ user_counts, const uint32_t* __restrict__ rows_dev = nullptr) {
+    const uint32_t* counts, uint32_t* prefix, uint32_t T, uint32_t ld, uint32_t chunk,
+    uint32_t* __restrict__ totals, uint32_t* __restrict__ user_counts,
+    const uint32_t* __restrict__ rows_dev) {
     static_assert(PER % 4 == 0, "vector loads");
     constexpr int Q = PER / 4;                   // 16-byte accesses per lane per pass
     constexpr uint32_t WSEG = kWave * PER;       // entries per wave per pass
+    constexpr int NWAVE = THREADS / kWave;
     // Row count from device memory (a planned queue): rows plus the zero after them.
     if (rows_dev) T = *rows_dev + 1u;
-    constexpr int NWAVE = THREADS / kWave;
+    const uint64_t lo64 = (uint64_t)blockIdx.y * chunk;
+    if (lo64 >= T) return;
+    const uint32_t lo = (uint32_t)lo64;
+    const uint32_t hi = (uint32_t)min((uint64_t)T, lo64 + chunk);
     __shared__ uint32_t wsum[NWAVE];
-    uint32_t* row = counts + (size_t)blockIdx.x * ld;
-    const uint32_t lane = lane_id(), wave = threadIdx.x / kWave;
+    const uint32_t* row = counts + (size_t)blockIdx.x * ld;
+    uint32_t* orow = prefix + (size_t)blockIdx.x * ld;
+    const uint32_t lane = lane_id(), wave = wave_id();
+
+    // The sum of the row before the chunk: 16-byte loads, four per lane in flight.
     uint32_t carry = 0;
-    for (uint32_t pbase = 0; pbase < T; pbase += THREADS * PER) {
+    if (lo) {
+        const u32x4* r4 = reinterpret_cast<const u32x4*>(row);
+        const uint32_t nq = lo / 4;
+        uint32_t part = 0, i = threadIdx.x;
+        for (; i + 3 * THREADS < nq; i += 4 * THREADS) {
+            const u32x4 a = r4[i], b = r4[i + THREADS], c = r4[i + 2 * THREADS],
+                        d = r4[i + 3 * THREADS];
+            part += ((a.x + a.y) + (a.z + a.w)) + ((b.x + b.y) + (b.z + b.w)) +
+                    ((c.x + c.y) + (c.z + c.w)) + ((d.x + d.y) + (d.z + d.w));
+        }
+        for (; i < nq; i += THREADS) {
+            const u32x4 a = r4[i];
+            part += (a.x + a.y) + (a.z + a.w);
+        }
+        part = wave_sum(part);
+        if (lane == 0) wsum[wave] = part;
+        __syncthreads();
+#pragma unroll
+        for (int w = 0; w < NWAVE; ++w) carry += wsum[w];
+        __syncthreads();
+    }
+    for (uint32_t pbase = lo; pbase < hi; pbase += THREADS * PER) {
         const uint32_t seg = pbase + wave * WSEG;
         u32x4 q[Q];
         uint32_t pre[Q];  // entries of the segment before this lane's four, per access
@@ -257,13 +291,13 @@ __global__ __launch_bounds__(THREADS) void k_tile_scan(
 #pragma unroll
         for (int j = 0; j < Q; ++j) {
             const uint32_t e0 = seg + j * (kWave * 4) + lane * 4;
-            if (e0 + 4 <= T) {
+            if (e0 + 4 <= hi) {
                 q[j] = *reinterpret_cast<const u32x4*>(row + e0);
             } else {
-                q[j].x = e0 < T ? row[e0] : 0u;
-                q[j].y = e0 + 1 < T ? row[e0 + 1] : 0u;
-                q[j].z = e0 + 2 < T ? row[e0 + 2] : 0u;
-                q[j].w = 0u;  // e0 + 3 >= T here
+                q[j].x = e0 < hi ? row[e0] : 0u;
+                q[j].y = e0 + 1 < hi ? row[e0 + 1] : 0u;
+                q[j].z = e0 + 2 < hi ? row[e0 + 2] : 0u;
+                q[j].w = 0u;  // e0 + 3 >= hi here
             }
         }
 #pragma unroll
@@ -289,21 +323,42 @@ __global__ __launch_bounds__(THREADS) void k_tile_scan(
             o.y = o.x + q[j].x;
             o.z = o.y + q[j].y;
             o.w = o.z + q[j].z;
-            if (e0 + 4 <= T) {
-                *reinterpret_cast<u32x4*>(row + e0) = o;
+            if (e0 + 4 <= hi) {
+                *reinterpret_cast<u32x4*>(orow + e0) = o;
             } else {
-                if (e0 < T) row[e0] = o.x;
-                if (e0 + 1 < T) row[e0 + 1] = o.y;
-                if (e0 + 2 < T) row[e0 + 2] = o.z;
+                if (e0 < hi) orow[e0] = o.x;
+                if (e0 + 1 < hi) orow[e0 + 1] = o.y;
+                if (e0 + 2 < hi) orow[e0 + 2] = o.z;
             }
         }
         carry += tot;
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0 && hi == T) {
         if (totals) totals[blockIdx.x] = carry;
         if (user_counts) user_counts[blockIdx.x] = carry;
     }
+}
+
+constexpr int kScanThreads = 1024;
+constexpr int kScanPer = 16;
+constexpr uint32_t kScanMaxChunks = 16;  // chunks per row: bounds the re-summed prefix
+
+// Launches the tile scan of `rows` class rows of at most t_max entries (the
+// entry count comes from rows_dev when given, t_max bounding it).  An
+// in-place scan (prefix == counts) runs one chunk per row.
+inline void launch_tile_scan(const uint32_t* counts, uint32_t* prefix, uint32_t t_max,
+                             uint32_t ld, uint32_t rows, uint32_t* totals, uint32_t* user_counts,
+                             const uint32_t* rows_dev, hipStream_t s) {
+    constexpr uint32_t kPass = (uint32_t)kScanThreads * kScanPer;
+    const uint32_t passes = (uint32_t)(((uint64_t)t_max + kPass - 1) / kPass);  // per row
+    uint32_t per_chunk = (passes + kScanMaxChunks - 1) / kScanMaxChunks;
+    if (prefix == counts || per_chunk == 0) per_chunk = passes ? passes : 1u;
+    const uint32_t chunk = per_chunk * kPass;
+    const uint32_t nchunks = passes ? (passes + per_chunk - 1) / per_chunk : 1u;
+    hipLaunchKernelGGL((k_tile_scan<kScanThreads, kScanPer>), dim3(rows, nchunks),
+                       dim3(kScanThreads), 0, s, counts, prefix, t_max, ld, chunk, totals,
+                       user_counts, rows_dev);
 }
 
 // ---------------------------------------------------------------------------

@@ -56,7 +56,9 @@ struct SegArgs {
     SegLong* longs;
     SegPlan* plan;
     uint32_t* counts;
+    uint32_t* prefix;   // the scan of counts
     uint32_t ld;
+    uint32_t max_rows;  // bound on the plan's row count
     uint32_t nseg;
     uint32_t mrows;
     uint32_t grid_cap;  // no more workgroups than the queue can hold
@@ -103,11 +105,11 @@ void launch_cfg(const SegArgs& a, hipStream_t s) {
         resident_per_cu(k_seg_partition<NW, R, NBITS, MASK, STAGED>, NW * kWave);
     hipLaunchKernelGGL((k_seg_count<NW, R, NBITS, MASK>), dim3(persistent_grid(count_per_cu, a)),
                        dim3(NW * kWave), 0, s, a.desc, a.items, a.plan, a.counts, a.ld);
-    hipLaunchKernelGGL((k_tile_scan<1024, 64>), dim3(a.mrows), dim3(1024), 0, s, a.counts, 0u,
-                       a.ld, (uint32_t*)nullptr, (uint32_t*)nullptr, &a.plan->n_rows);
+    launch_tile_scan(a.counts, a.prefix, a.max_rows + 1u, a.ld, a.mrows, nullptr, nullptr,
+                     &a.plan->n_rows, s);
     hipLaunchKernelGGL((k_seg_partition<NW, R, NBITS, MASK, STAGED>),
                        dim3(persistent_grid(part_per_cu, a)), dim3(NW * kWave), 0, s, a.desc,
-                       a.items, a.plan, a.counts, a.ld);
+                       a.items, a.plan, a.prefix, a.ld);
 }
 
 template <int NBITS, bool MASK>
@@ -150,9 +152,12 @@ bool config_available(bool mask, uint32_t nbits, int cfg) {
 
 // ---- per-context state: tuning table and captured graphs ------------------
 
+// A captured graph holds every pointer it was captured with, the context
+// workspace's included: the key carries the workspace base, so that a grown
+// (reallocated) workspace is captured anew instead of replayed freed.
 struct GraphKey {
     std::tuple<const void*, const void*, const void*, const void*, uint64_t, uint64_t, uint32_t,
-               uint32_t, int, void*>
+               uint32_t, int, void*, const void*>
         k;
     bool operator<(const GraphKey& o) const { return k < o.k; }
 };
@@ -239,7 +244,9 @@ struct SegWorkspace {
     SegLong* longs;
     SegPlan* plan;
     uint32_t* counts;
+    uint32_t* prefix;
     uint32_t ld;
+    uint32_t max_rows;
 };
 
 // Workspace for any candidate configuration (sized by the smallest tile).
@@ -259,6 +266,7 @@ mdiscr_status seg_workspace(mdiscr_ctx* ctx, uint64_t n, uint64_t nseg, uint32_t
     const size_t o_longs = take(nseg * sizeof(SegLong));
     const size_t o_plan = take(sizeof(SegPlan));
     const size_t o_counts = take((size_t)m * ld * sizeof(uint32_t));
+    const size_t o_prefix = take((size_t)m * ld * sizeof(uint32_t));
     void* base = nullptr;
     mdiscr_status s = ctx_workspace(ctx, off, &base);
     if (s) return s;
@@ -268,7 +276,9 @@ mdiscr_status seg_workspace(mdiscr_ctx* ctx, uint64_t n, uint64_t nseg, uint32_t
     w.longs = reinterpret_cast<SegLong*>(b + o_longs);
     w.plan = reinterpret_cast<SegPlan*>(b + o_plan);
     w.counts = reinterpret_cast<uint32_t*>(b + o_counts);
+    w.prefix = reinterpret_cast<uint32_t*>(b + o_prefix);
     w.ld = (uint32_t)ld;
+    w.max_rows = (uint32_t)max_rows;
     return MDISCR_OK;
 }
 
@@ -290,7 +300,9 @@ mdiscr_status run_segmented(mdiscr_ctx* ctx, const uint32_t* d_in, uint32_t* d_o
     a.longs = w.longs;
     a.plan = w.plan;
     a.counts = w.counts;
+    a.prefix = w.prefix;
     a.ld = w.ld;
+    a.max_rows = w.max_rows;
     a.nseg = (uint32_t)nseg;
     a.mrows = d.m;
     a.grid_cap = grid_cap(2 * nseg + n / kMinTile + 2);
@@ -308,7 +320,7 @@ mdiscr_status run_segmented(mdiscr_ctx* ctx, const uint32_t* d_in, uint32_t* d_o
     if (allow_graph && s != nullptr && !ctx->timing) {
         GraphKey key{std::make_tuple((const void*)d_in, (const void*)d_out, (const void*)d_seg_off,
                                      (const void*)d_seg_counts, n, nseg, d.m, d.kind, cfg,
-                                     (void*)s)};
+                                     (void*)s, (const void*)w.desc)};
         auto it = ss->graphs.find(key);
         if (it == ss->graphs.end()) {
             if (ss->graphs.size() >= 32) {
@@ -546,8 +558,7 @@ mdiscr_status mdiscr_segm_flags(mdiscr_ctx* ctx, uint64_t n, const uint32_t* d_i
     hipLaunchKernelGGL(k_flag_count, dim3(nb), dim3(kFlagBlock), 0, s, d_segment_sizes,
                        (uint32_t)n, bc);
     // exclusive scan of the block counts; entry nb becomes the segment count
-    hipLaunchKernelGGL((k_tile_scan<1024, 64>), dim3(1), dim3(1024), 0, s, bc, nb + 1, nbl,
-                       (uint32_t*)nullptr, (uint32_t*)nullptr, (const uint32_t*)nullptr);
+    launch_tile_scan(bc, bc, nb + 1, nbl, 1u, nullptr, nullptr, nullptr, s);
     hipLaunchKernelGGL(k_flag_scatter, dim3(nb), dim3(kFlagBlock), 0, s, d_segment_sizes,
                        (uint32_t)n, bc, off, bad);
     uint32_t nseg = 0;
@@ -654,7 +665,9 @@ mdiscr_status mdiscr_partition_batch(mdiscr_ctx* ctx, const mdiscr_request* reqs
     a.longs = w.longs;
     a.plan = w.plan;
     a.counts = w.counts;
+    a.prefix = w.prefix;
     a.ld = w.ld;
+    a.max_rows = w.max_rows;
     a.nseg = nreq;
     a.mrows = mmax;
     {   // as many workgroups as the queue will hold, up to the resident capacity

@@ -61,7 +61,8 @@ mdiscr_status launch_soa(mdiscr_ctx* ctx, const SoaCols& cols, Src src, uint32_t
     const uint32_t Kc = (T + kCCT - 1) / kCCT;
     const uint32_t Kp = (T + kCT - 1) / kCT;
     uint32_t* counts = ws;
-    uint32_t* totals = ws + (size_t)m * ld;
+    uint32_t* prefix = ws + (size_t)m * ld;
+    uint32_t* totals = ws + (size_t)2 * m * ld;
     TimedLaunch tl;
     timing_begin(ctx, s, tl);
     bool vec = false;
@@ -77,12 +78,11 @@ mdiscr_status launch_soa(mdiscr_ctx* ctx, const SoaCols& cols, Src src, uint32_t
     }
     timing_end(ctx, s, tl, "soa_count");
     timing_begin(ctx, s, tl);
-    hipLaunchKernelGGL((k_tile_scan<1024, 64>), dim3(m), dim3(1024), 0, s, counts, T, ld, totals,
-                       user_counts, (const uint32_t*)nullptr);
+    launch_tile_scan(counts, prefix, T, ld, m, totals, user_counts, nullptr, s);
     timing_end(ctx, s, tl, "soa_scan");
     timing_begin(ctx, s, tl);
     hipLaunchKernelGGL((k_partition_soa<NW, R, kCT, NBITS, Src, ST, Discr>), dim3(Kp),
-                       dim3(NW * kWave), 0, s, cols, src, n, dis, counts, totals, ld, m);
+                       dim3(NW * kWave), 0, s, cols, src, n, dis, prefix, totals, ld, m);
     timing_end(ctx, s, tl, "soa_partition");
     return ctx_hip(ctx, hipGetLastError(), "SoA partition launch");
 }
@@ -169,7 +169,7 @@ mdiscr_status run_soa(mdiscr_ctx* ctx, SoaCols& cols, uint64_t n, const ClassSpe
     const uint64_t T = (n + tile - 1) / tile;
     const uint64_t ld = (T + 3) & ~3ull;
     void* ws = nullptr;
-    st = ctx_workspace(ctx, (ld + 1) * m * sizeof(uint32_t), &ws);
+    st = ctx_workspace(ctx, (2 * ld + 1) * m * sizeof(uint32_t), &ws);  // counts, prefix, totals
     if (st) return st;
     uint32_t* w = static_cast<uint32_t*>(ws);
     return wide ? run_staged<uint64_t>(ctx, cols, (uint32_t)n, cs, d_counts, s, w)

@@ -134,6 +134,44 @@ def test_graph_replay_sees_new_contents(ctx):
         assert first_diff(from_dev(d_out), ref) < 0, f"replay {rep}"
 
 
+def test_graph_replay_after_workspace_growth():
+    """A call captured into a graph, then a larger call on the same context
+    (which reallocates the shared workspace), then the first call again: the
+    graph captured against the old workspace must not be replayed."""
+    import torch
+    import mdiscriminator_amd as md
+    c = md.Context()   # its own, small workspace
+    try:
+        L = lengths_mix(7, 300, 2000)
+        off = W.offsets_from_lengths(L)
+        n = int(off[-1])
+        s = torch.cuda.Stream()
+        d_in = torch.empty(n, dtype=torch.int32, device="cuda:0")
+        d_out = torch.empty_like(d_in)
+        d_off = torch.from_numpy(off.astype(np.int64)).to("cuda:0")
+        big = W.uniform_u31(1 << 24, 77)
+        d_big = to_dev(big)
+        d_big_out = torch.empty_like(d_big)
+        for rep in range(3):
+            x = W.uniform_u31(n, 2000 + rep)
+            d_in.copy_(torch.from_numpy(x.view(np.int32)))
+            torch.cuda.synchronize()
+            print(f"workspace growth replay {rep}", flush=True)
+            with torch.cuda.stream(s):
+                c.partition_segmented(d_in, d_out, d_off, md.Mod4, stream=s)
+            s.synchronize()
+            ref, _ = oracle.seq_mdiscr_segm(x, off, 4)
+            assert first_diff(from_dev(d_out), ref) < 0, f"call {rep}"
+            if rep == 0:
+                # grows the context's workspace well past the segmented call's
+                c.partition_u32(d_big, d_big_out, md.Mod(7))
+                torch.cuda.synchronize()
+                ref_big, _ = oracle.seq_mdiscr(big, 7)
+                assert first_diff(from_dev(d_big_out), ref_big) < 0
+    finally:
+        c.close()
+
+
 def test_flags_interface_random(ctx):
     import torch
     import mdiscriminator_amd as md
