@@ -191,6 +191,25 @@ def main():
     ktz = kernel_times(ctx, step_zipf, max(args.steps, 10), kernels)
     part_ms = kt["partition"]
     achieved = N * BYTES_PER_ELEM / (part_ms * 1e-3) / 1e9
+
+    # The practical ceiling for the partition's traffic on this device: an
+    # elementwise copy (torch.neg) of the same buffers, 1 GiB read + 1 GiB
+    # written, median of 20 event-timed launches.
+    f_in, f_out = d_in.view(torch.float32), d_out.view(torch.float32)
+    for _ in range(3):
+        torch.neg(f_in, out=f_out)
+    torch.cuda.synchronize()
+    copy_ms = []
+    for _ in range(20):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        torch.neg(f_in, out=f_out)
+        e1.record()
+        torch.cuda.synchronize()
+        copy_ms.append(e0.elapsed_time(e1))
+    copy_ms = sorted(copy_ms)[len(copy_ms) // 2]
+    copy_gbs = N * BYTES_PER_ELEM / (copy_ms * 1e-3) / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_partition_2p28_m8.json")
     if os.path.exists(tpath):
@@ -294,6 +313,11 @@ def main():
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": N * BYTES_PER_ELEM,
                      "avg_launch_ms": round(part_ms, 5)},
+        "copy_ceiling": {"kernel": "elementwise copy of the same buffers (torch.neg), "
+                                   "1 GiB read + 1 GiB written",
+                         "ms": round(copy_ms, 5), "achieved": round(copy_gbs, 1), "unit": "GB/s",
+                         "frac_of_peak": round(copy_gbs / PEAK_HBM_GBS, 4),
+                         "partition_vs_copy": round(achieved / copy_gbs, 4)},
         "kernels_ms": {k: round(v, 5) for k, v in kt.items() if v},
         "zipf_variant": {"value": round(N * args.steps / dtz / 1e9, 3), "unit": "Gelem/s",
                          "kernels_ms": {k: round(v, 5) for k, v in ktz.items() if v}},
