@@ -183,3 +183,40 @@ def test_full_size_2p28_m8(ctx, variant):
     out = from_dev(d_out)
     i = first_diff(out, ref)
     assert i < 0, f"first mismatch at {i}"
+
+
+@pytest.mark.parametrize("m", [8, 7])
+def test_maximum_size(ctx, m):
+    """n = MDISCR_MAX_ELEMS (0xFFFF0000, about 4.29 G elements, 17 GB each way)
+    with x[i] = i, so element offsets, tile indexes and values cross 2^31 and
+    run up to the 32-bit limit.  The exact output has a closed form: class k
+    holds k, k + m, k + 2m, ... in order, so out[off_k + j] = m*j + k, checked
+    on the device chunk by chunk (placement, stability and the permutation)."""
+    import torch
+    import mdiscriminator_amd as md
+    n = 0xFFFF0000
+    chunk = 1 << 27
+    print(f"max size m={m}: generating", flush=True)
+    d_in = torch.empty(n, dtype=torch.int32, device="cuda:0")
+    for s in range(0, n, chunk):
+        e = min(n, s + chunk)
+        d_in[s:e] = torch.arange(s, e, dtype=torch.int64, device="cuda:0").to(torch.int32)
+    d_out = torch.empty_like(d_in)
+    d_cnt = torch.empty(m, dtype=torch.int32, device="cuda:0")
+    discr = md.Mask(m) if m & (m - 1) == 0 else md.Mod(m)
+    print(f"max size m={m}: partition", flush=True)
+    ctx.partition_u32(d_in, d_out, discr, d_counts=d_cnt)
+    torch.cuda.synchronize()
+    cnt = from_dev(d_cnt).astype(np.int64)
+    want = np.array([(n - k + m - 1) // m for k in range(m)], dtype=np.int64)
+    assert np.array_equal(cnt, want)
+    print(f"max size m={m}: checking", flush=True)
+    off = 0
+    for k in range(m):
+        for j0 in range(0, int(want[k]), chunk):
+            j1 = min(int(want[k]), j0 + chunk)
+            exp = (torch.arange(j0, j1, dtype=torch.int64, device="cuda:0") * m + k).to(torch.int32)
+            assert torch.equal(d_out[off + j0:off + j1], exp), f"class {k}, j in [{j0}, {j1})"
+        off += int(want[k])
+    del d_in, d_out
+    torch.cuda.empty_cache()

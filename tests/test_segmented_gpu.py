@@ -258,3 +258,43 @@ def test_full_size_10k_lognormal_flags(ctx):
     ref, ref_new = oracle.segm_flags(x, f, 4)
     assert first_diff(from_dev(d_out), ref) < 0
     assert first_diff(d_new.cpu().numpy(), ref_new) < 0
+
+
+def test_segmented_maximum_size(ctx):
+    """n = MDISCR_MAX_ELEMS in three segments, the first past 2^31 elements long,
+    x[i] = i, m = 4: in segment [a, b), class k holds the i = k (mod 4) in
+    order, so its j-th element is first_k + 4j, checked on the device."""
+    import torch
+    import mdiscriminator_amd as md
+    n, m = 0xFFFF0000, 4
+    off = np.array([0, (1 << 31) + 5, 0xF0000003, n], dtype=np.uint64)
+    chunk = 1 << 27
+    print("segmented max size: generating", flush=True)
+    d_in = torch.empty(n, dtype=torch.int32, device="cuda:0")
+    for s in range(0, n, chunk):
+        e = min(n, s + chunk)
+        d_in[s:e] = torch.arange(s, e, dtype=torch.int64, device="cuda:0").to(torch.int32)
+    d_out = torch.empty_like(d_in)
+    d_off = torch.from_numpy(off.astype(np.int64)).to("cuda:0")
+    d_cnt = torch.empty(3 * m, dtype=torch.int32, device="cuda:0")
+    print("segmented max size: partition", flush=True)
+    ctx.partition_segmented(d_in, d_out, d_off, md.Mod4, h_seg_off=off, d_seg_counts=d_cnt)
+    torch.cuda.synchronize()
+    cnt = d_cnt.cpu().numpy().astype(np.int64).reshape(3, m)
+    print("segmented max size: checking", flush=True)
+    for sg in range(3):
+        a, b = int(off[sg]), int(off[sg + 1])
+        pos = a
+        for k in range(m):
+            first = a + ((k - a) % m)
+            count = (b - first + m - 1) // m if first < b else 0
+            assert cnt[sg, k] == count, f"segment {sg} class {k} count"
+            for j0 in range(0, count, chunk):
+                j1 = min(count, j0 + chunk)
+                exp = (torch.arange(j0, j1, dtype=torch.int64, device="cuda:0") * m + first)
+                assert torch.equal(d_out[pos + j0:pos + j1], exp.to(torch.int32)), \
+                    f"segment {sg} class {k}"
+            pos += count
+        assert pos == b
+    del d_in, d_out
+    torch.cuda.empty_cache()

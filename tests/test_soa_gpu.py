@@ -264,3 +264,39 @@ def test_byte_columns_misaligned(ctx):
             for j, (o, r) in enumerate(zip(d_out, ref)):
                 assert first_diff(host(o, cols[j]), r) < 0, \
                     f"in+{off_in} out+{off_out} key {key_col} column {j}"
+
+
+def test_soa_maximum_size(ctx):
+    """n = MDISCR_MAX_ELEMS records of (u32 key = i, u8 payload = (i >> 3) & 0xFF),
+    m = 16: record indexes cross 2^31.  Closed form: class k holds k, k + 16,
+    ... in order, so key_out[off_k + j] = 16j + k and the payload follows it."""
+    import torch
+    import mdiscriminator_amd as md
+    n, m = 0xFFFF0000, 16
+    chunk = 1 << 27
+    print("soa max size: generating", flush=True)
+    d_key = torch.empty(n, dtype=torch.int32, device="cuda:0")
+    d_pay = torch.empty(n, dtype=torch.uint8, device="cuda:0")
+    for s in range(0, n, chunk):
+        e = min(n, s + chunk)
+        i = torch.arange(s, e, dtype=torch.int64, device="cuda:0")
+        d_key[s:e] = i.to(torch.int32)
+        d_pay[s:e] = ((i >> 3) & 0xFF).to(torch.uint8)
+    o_key, o_pay = torch.empty_like(d_key), torch.empty_like(d_pay)
+    d_cnt = torch.empty(m, dtype=torch.int32, device="cuda:0")
+    print("soa max size: partition", flush=True)
+    ctx.partition_soa([d_key, d_pay], [o_key, o_pay], md.Mask(m), d_counts=d_cnt)
+    torch.cuda.synchronize()
+    assert np.array_equal(d_cnt.cpu().numpy().astype(np.int64), np.full(m, n // m))
+    print("soa max size: checking", flush=True)
+    per = n // m
+    for k in range(m):
+        for j0 in range(0, per, chunk):
+            j1 = min(per, j0 + chunk)
+            i = torch.arange(j0, j1, dtype=torch.int64, device="cuda:0") * m + k
+            a = k * per
+            assert torch.equal(o_key[a + j0:a + j1], i.to(torch.int32)), f"key class {k}"
+            assert torch.equal(o_pay[a + j0:a + j1], ((i >> 3) & 0xFF).to(torch.uint8)), \
+                f"payload class {k}"
+    del d_key, d_pay, o_key, o_pay
+    torch.cuda.empty_cache()
