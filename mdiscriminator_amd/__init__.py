@@ -111,12 +111,36 @@ class Pred:
         return _lib.MdiscrPred(self.op, self.a & 0xFFFFFFFF, self.b & 0xFFFFFFFF)
 
 
-def _ptr(t):
+def _ptr(t, nbytes=None):
+    """Device address of a tensor argument; an int is taken as an address and
+    passed through unchecked.  The kernels read and write dense arrays from the
+    address given, so a tensor must be a contiguous GPU tensor of at least
+    nbytes: a host tensor, a strided view or a short buffer raises here instead
+    of faulting or corrupting memory on the device."""
     if t is None:
         return None
     if isinstance(t, int):
         return t
+    if not getattr(t, "is_cuda", False):
+        raise MdiscrError(_lib.EINVAL, f"expected a GPU tensor, got one on {t.device}")
+    if not t.is_contiguous():
+        raise MdiscrError(_lib.EINVAL, "expected a contiguous tensor")
+    have = t.numel() * t.element_size()
+    if nbytes is not None and have < nbytes:
+        raise MdiscrError(_lib.EINVAL, f"tensor of {have} bytes, the call needs {nbytes}")
     return t.data_ptr()
+
+
+def _columns(cols, out_cols, n):
+    """Pointer arrays and element sizes of SoA columns, each checked to hold
+    n elements (an output column has its input's element size)."""
+    k = len(cols)
+    if len(out_cols) != k:
+        raise MdiscrError(_lib.EINVAL, f"{k} input columns but {len(out_cols)} output columns")
+    sizes = [c.element_size() for c in cols]
+    pin = (ctypes.c_void_p * k)(*[_ptr(c, n * b) for c, b in zip(cols, sizes)])
+    pout = (ctypes.c_void_p * k)(*[_ptr(c, n * b) for c, b in zip(out_cols, sizes)])
+    return pin, pout, (ctypes.c_uint32 * k)(*sizes)
 
 
 def _stream(stream):
@@ -160,15 +184,16 @@ class Context:
     def partition_u32(self, d_in, d_out, discr, d_counts=None, n=None, stream=None):
         """Stream-ordered stable partition of d_in into d_out (uint32/int32 tensors)."""
         n = d_in.numel() if n is None else n
-        st = self.lib.mdiscr_partition_u32(self.h, _ptr(d_in), _ptr(d_out), n, discr.c(),
-                                           _ptr(d_counts), _stream(stream))
+        st = self.lib.mdiscr_partition_u32(self.h, _ptr(d_in, 4 * n), _ptr(d_out, 4 * n), n,
+                                           discr.c(), _ptr(d_counts, 4 * discr.m),
+                                           _stream(stream))
         self._check(st, "mdiscr_partition_u32")
 
     def mdiscr(self, discr, num_elems, num_hwd_thds, d_in, d_out):
         """Reference-shaped synchronous call; returns the m class counts."""
         counts = (ctypes.c_uint32 * discr.m)()
-        st = self.lib.mdiscr_u32(self.h, num_elems, num_hwd_thds, _ptr(d_in), _ptr(d_out),
-                                 discr.c(), counts)
+        st = self.lib.mdiscr_u32(self.h, num_elems, num_hwd_thds, _ptr(d_in, 4 * num_elems),
+                                 _ptr(d_out, 4 * num_elems), discr.c(), counts)
         self._check(st, "mdiscr_u32")
         return list(counts)
 
@@ -179,11 +204,9 @@ class Context:
         class of cols[key_col] (a 32-bit column); payloads move as raw bits."""
         k = len(cols)
         n = cols[key_col].numel() if n is None else n
-        pin = (ctypes.c_void_p * k)(*[_ptr(c) for c in cols])
-        pout = (ctypes.c_void_p * k)(*[_ptr(c) for c in out_cols])
-        nb = (ctypes.c_uint32 * k)(*[c.element_size() for c in cols])
+        pin, pout, nb = _columns(cols, out_cols, n)
         st = self.lib.mdiscr_partition_soa(self.h, k, pin, nb, key_col, pout, n, discr.c(),
-                                           _ptr(d_counts), _stream(stream))
+                                           _ptr(d_counts, 4 * discr.m), _stream(stream))
         self._check(st, "mdiscr_partition_soa")
 
     def partition_soa_select(self, cols, out_cols, discr, sel_col, a_col, b_col, d_counts=None,
@@ -192,11 +215,10 @@ class Context:
         ModTuple body, MdiscrHost_tuples.cu.h:27,46); 1- or 4-byte columns."""
         k = len(cols)
         n = cols[0].numel() if n is None else n
-        pin = (ctypes.c_void_p * k)(*[_ptr(c) for c in cols])
-        pout = (ctypes.c_void_p * k)(*[_ptr(c) for c in out_cols])
-        nb = (ctypes.c_uint32 * k)(*[c.element_size() for c in cols])
+        pin, pout, nb = _columns(cols, out_cols, n)
         st = self.lib.mdiscr_partition_soa_select(self.h, k, pin, nb, sel_col, a_col, b_col, pout,
-                                                  n, discr.c(), _ptr(d_counts), _stream(stream))
+                                                  n, discr.c(), _ptr(d_counts, 4 * discr.m),
+                                                  _stream(stream))
         self._check(st, "mdiscr_partition_soa_select")
 
     # -- two-way filter ----------------------------------------------------------
@@ -204,19 +226,17 @@ class Context:
         """parFilter (ParFilter.hs:8-21): kept elements, then the rest, both in
         input order; d_counts (2 entries) = [kept, rest]."""
         n = d_in.numel() if n is None else n
-        st = self.lib.mdiscr_filter_u32(self.h, _ptr(d_in), _ptr(d_out), n, pred.c(),
-                                        _ptr(d_counts), _stream(stream))
+        st = self.lib.mdiscr_filter_u32(self.h, _ptr(d_in, 4 * n), _ptr(d_out, 4 * n), n,
+                                        pred.c(), _ptr(d_counts, 8), _stream(stream))
         self._check(st, "mdiscr_filter_u32")
 
     def filter_soa(self, cols, out_cols, pred, key_col=0, d_counts=None, n=None, stream=None):
         """The filter over records: predicate on the 4-byte key column."""
         k = len(cols)
         n = cols[key_col].numel() if n is None else n
-        pin = (ctypes.c_void_p * k)(*[_ptr(c) for c in cols])
-        pout = (ctypes.c_void_p * k)(*[_ptr(c) for c in out_cols])
-        nb = (ctypes.c_uint32 * k)(*[c.element_size() for c in cols])
+        pin, pout, nb = _columns(cols, out_cols, n)
         st = self.lib.mdiscr_filter_soa(self.h, k, pin, nb, key_col, pout, n, pred.c(),
-                                        _ptr(d_counts), _stream(stream))
+                                        _ptr(d_counts, 8), _stream(stream))
         self._check(st, "mdiscr_filter_soa")
 
     # -- array-valued elements --------------------------------------------------
@@ -224,8 +244,9 @@ class Context:
         """Elements of `length` uint32 words stored flat; class from word 0
         (MdiscrHost_arrays.cu.h:14-96)."""
         n = d_in.numel() // length if n is None else n
-        st = self.lib.mdiscr_partition_arrays(self.h, _ptr(d_in), _ptr(d_out), n, length,
-                                              discr.c(), _ptr(d_counts), _stream(stream))
+        st = self.lib.mdiscr_partition_arrays(self.h, _ptr(d_in, 4 * n * length),
+                                              _ptr(d_out, 4 * n * length), n, length, discr.c(),
+                                              _ptr(d_counts, 4 * discr.m), _stream(stream))
         self._check(st, "mdiscr_partition_arrays")
 
     # -- segmented -------------------------------------------------------------
@@ -236,15 +257,18 @@ class Context:
         hp = None
         if h_seg_off is not None:
             hp = h_seg_off.ctypes.data_as(ctypes.c_void_p)
-        st = self.lib.mdiscr_partition_segmented(self.h, _ptr(d_in), _ptr(d_out), d_in.numel(),
-                                                 _ptr(d_seg_off), hp, nseg, discr.c(),
-                                                 _ptr(d_seg_counts), _stream(stream))
+        n = d_in.numel()
+        st = self.lib.mdiscr_partition_segmented(self.h, _ptr(d_in, 4 * n), _ptr(d_out, 4 * n), n,
+                                                 _ptr(d_seg_off, 8 * (nseg + 1)), hp, nseg,
+                                                 discr.c(), _ptr(d_seg_counts, 4 * nseg * discr.m),
+                                                 _stream(stream))
         self._check(st, "mdiscr_partition_segmented")
 
     def segm_flags(self, discr, num_elems, d_in, d_segment_sizes, d_out, d_new_sizes,
                    stream=None):
-        st = self.lib.mdiscr_segm_flags(self.h, num_elems, _ptr(d_in), _ptr(d_segment_sizes),
-                                        _ptr(d_out), _ptr(d_new_sizes), discr.c(),
+        nb = 4 * num_elems
+        st = self.lib.mdiscr_segm_flags(self.h, num_elems, _ptr(d_in, nb), _ptr(d_segment_sizes, nb),
+                                        _ptr(d_out, nb), _ptr(d_new_sizes, nb), discr.c(),
                                         _stream(stream))
         self._check(st, "mdiscr_segm_flags")
 
@@ -253,15 +277,16 @@ class Context:
         arr = (_lib.MdiscrRequest * len(requests))()
         for i, r in enumerate(requests):
             d_in, d_out, discr = r[0], r[1], r[2]
-            arr[i] = _lib.MdiscrRequest(_ptr(d_in), _ptr(d_out), d_in.numel(), discr.c(),
-                                        _ptr(r[3]) if len(r) > 3 else None)
+            n = d_in.numel()
+            arr[i] = _lib.MdiscrRequest(_ptr(d_in, 4 * n), _ptr(d_out, 4 * n), n, discr.c(),
+                                        _ptr(r[3], 4 * discr.m) if len(r) > 3 else None)
         st = self.lib.mdiscr_partition_batch(self.h, arr, len(requests), _stream(stream))
         self._check(st, "mdiscr_partition_batch")
 
     # -- output format ---------------------------------------------------------
     def sizes_from_counts(self, d_counts, m, n, d_sizes, stream=None):
-        st = self.lib.mdiscr_sizes_from_counts(self.h, _ptr(d_counts), m, n, _ptr(d_sizes),
-                                               _stream(stream))
+        st = self.lib.mdiscr_sizes_from_counts(self.h, _ptr(d_counts, 4 * m), m, n,
+                                               _ptr(d_sizes, 4 * n), _stream(stream))
         self._check(st, "mdiscr_sizes_from_counts")
 
     # -- timing ----------------------------------------------------------------
@@ -310,7 +335,8 @@ class Server:
     def submit(self, d_in, discr, n=None):
         t = ctypes.c_uint64()
         n = d_in.numel() if n is None else n
-        st = self.lib.mdiscr_server_submit(self.h, _ptr(d_in), n, discr.c(), ctypes.byref(t))
+        st = self.lib.mdiscr_server_submit(self.h, _ptr(d_in, 4 * n), n, discr.c(),
+                                           ctypes.byref(t))
         if st != _lib.OK:
             raise MdiscrError(st, "mdiscr_server_submit")
         self._m[t.value] = (discr.m, n)

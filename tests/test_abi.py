@@ -50,3 +50,14 @@ def test_status_strings():
     lib = _lib.load()
     for s in range(6):
         assert lib.mdiscr_status_str(s)
+
+
+def test_host_tensors_rejected_before_any_call():
+    """The Python layer hands the library device addresses only: a host tensor
+    raises before any call, so no host pointer reaches a kernel."""
+    import torch
+    import mdiscriminator_amd as md
+    with pytest.raises(md.MdiscrError) as e:
+        md._ptr(torch.zeros(4, dtype=torch.int32))
+    assert e.value.status == _lib.EINVAL
+    assert md._ptr(None) is None and md._ptr(1234) == 1234

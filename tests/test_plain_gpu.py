@@ -138,8 +138,16 @@ def test_invalid_arguments(ctx):
         assert e.value.status == 1
     with pytest.raises(md.MdiscrError):
         ctx.partition_u32(d, d[10:], md.Mod4, n=990)          # overlapping output
+    with pytest.raises(md.MdiscrError):                       # n beyond the limit, seen by
+        ctx.partition_u32(d.data_ptr(), o.data_ptr(), md.Mod4, n=1 << 40)   # the library
+    # tensors are checked before the call: strided, short, or short counts
     with pytest.raises(md.MdiscrError):
-        ctx.partition_u32(d, o, md.Mod4, n=1 << 40)           # n beyond the limit
+        ctx.partition_u32(d[::2], o[:500], md.Mod4)
+    with pytest.raises(md.MdiscrError):
+        ctx.partition_u32(d, o[:10], md.Mod4)
+    with pytest.raises(md.MdiscrError):
+        ctx.partition_u32(d, o, md.Mod(16), d_counts=torch.zeros(8, dtype=torch.int32,
+                                                                 device="cuda:0"))
     # the context still works after rejected calls
     x = W.uniform_u31(1000, 1)
     check(ctx, x, md.Mod4, "after errors")
