@@ -46,6 +46,34 @@ def test_batch_matches_alone_and_oracle(ctx):
             assert torch.equal(alone, d_out[i])
 
 
+def test_batch_sequence_on_stream(ctx):
+    """Successive batches on one non-null stream (the server's case), reusing
+    the context's pinned descriptor staging: other tensors, request counts,
+    sizes, class counts and contents in turn, each equal to the oracle,
+    counts included."""
+    import torch
+    import mdiscriminator_amd as md
+    s = torch.cuda.Stream()
+    plans = [(7, 3, 50000), (7, 4, 120000), (5, 5, 80000), (70, 6, 30000), (7, 7, 90000),
+             (3, 8, 2000000), (7, 9, 60000), (7, 10, 60000), (1, 11, 5000)]
+    for count, seed, hi in plans:
+        xs, ms = make_requests(100 + seed, count, hi=hi)
+        print(f"batch on stream: {count} requests, seed {seed}", flush=True)
+        d_in = [to_dev(x) if len(x) else torch.empty(0, dtype=torch.int32, device="cuda:0")
+                for x in xs]
+        d_out = [torch.full_like(t, -1) for t in d_in]
+        d_cnt = [torch.full((m,), -1, dtype=torch.int32, device="cuda:0") for m in ms]
+        torch.cuda.synchronize()
+        with torch.cuda.stream(s):
+            ctx.partition_batch([(d_in[i], d_out[i], md.Mod(ms[i]), d_cnt[i])
+                                 for i in range(count)], stream=s)
+        s.synchronize()
+        for i, x in enumerate(xs):
+            ref, ref_cnt = oracle.seq_mdiscr(x, ms[i])
+            assert first_diff(from_dev(d_out[i]), ref) < 0, f"seed {seed} request {i}"
+            assert np.array_equal(from_dev(d_cnt[i]), ref_cnt), f"seed {seed} request {i}"
+
+
 def test_batch_power_of_two_mix_uses_mask_kernels(ctx):
     import torch
     import mdiscriminator_amd as md
