@@ -3,9 +3,9 @@
 // mdiscr_partition_segmented: segments of one array given by offsets.
 // mdiscr_segm_flags: the reference's flag-array interface, mdiscrSegm(n, in,
 //   segment_sizes, out, new_sizes) (MdiscrSegmHost.cu.h:36-42).
-// Both run descriptor -> plan -> count -> scan -> partition.  A call on a
-// non-null stream is captured once into a HIP graph (keyed by its arguments)
-// and replayed afterwards.  The (block size, elements per thread, staged or
+// Both run descriptor -> plan -> count -> scan -> partition.  An offsets call
+// on a non-null stream is captured once into a HIP graph (keyed by its
+// arguments and the context workspace) and replayed afterwards.  The (block size, elements per thread, staged or
 // direct scatter) configuration is picked by timing every candidate on the
 // first call of each (discriminator kind, class bits, log2 size) bucket;
 // choices persist in the file named by MDISCR_TUNE_CACHE when set.
