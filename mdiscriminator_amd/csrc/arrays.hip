@@ -30,6 +30,9 @@ namespace {
 #ifndef MDISCR_ROWS_NW
 #define MDISCR_ROWS_NW 4
 #endif
+#ifndef MDISCR_ROWS_COUNT_CT
+#define MDISCR_ROWS_COUNT_CT 8
+#endif
 #ifndef MDISCR_ROWS_WPT
 #define MDISCR_ROWS_WPT 32
 #endif
@@ -140,7 +143,7 @@ mdiscr_status launch_rows(mdiscr_ctx* ctx, const uint32_t* in, uint32_t* out, ui
                           uint32_t* ws) {
     constexpr int R = rounds_for(LEN);
     constexpr uint32_t TILE = kNW * kWave * R;
-    constexpr int kCCT = 8;  // tiles per count workgroup
+    constexpr int kCCT = MDISCR_ROWS_COUNT_CT;  // tiles per count workgroup
     const uint32_t T = (uint32_t)(((uint64_t)n + TILE - 1) / TILE);
     const uint32_t ld = (T + 3u) & ~3u;
     uint32_t* counts = ws;

@@ -23,7 +23,6 @@ namespace {
 // rounds 0.416 ms, 12 rounds 0.403 ms, 16 rounds 0.426 ms).  16 rounds at 8
 // waves loses 20-30% (occupancy).
 constexpr int kCT = 1;    // tiles per partition workgroup
-constexpr int kCCT = 8;   // tiles per count workgroup
 
 #ifndef MDISCR_SOA_R
 #define MDISCR_SOA_R 12
@@ -38,6 +37,10 @@ struct SoaGeom {
     static constexpr int R =
         kLarge ? MDISCR_SOA64_R : (sizeof(ST) == 4 && NBITS >= 3 ? MDISCR_SOA_R : 8);
     static constexpr uint32_t TILE = NW * kWave * R;
+    // Tiles per count workgroup (2^26 keys, interleaved A/B, count ms for 1 /
+    // 2 / 8 tiles: m=16 0.042 / 0.045 / 0.051, select m=8 0.066 / 0.069 /
+    // 0.079, m=64 0.050 / 0.046 / 0.048).
+    static constexpr int CCT = kLarge ? 2 : 1;
 };
 
 // The tile launch_soa uses for these columns and classes (the workspace size).
@@ -55,6 +58,7 @@ template <int NBITS, typename ST, class Src, class Discr>
 mdiscr_status launch_soa(mdiscr_ctx* ctx, const SoaCols& cols, Src src, uint32_t n, Discr dis,
                          uint32_t m, uint32_t* user_counts, hipStream_t s, uint32_t* ws) {
     constexpr int NW = SoaGeom<ST, NBITS>::NW, R = SoaGeom<ST, NBITS>::R;
+    constexpr int kCCT = SoaGeom<ST, NBITS>::CCT;
     constexpr uint32_t TILE = SoaGeom<ST, NBITS>::TILE;
     const uint32_t T = (uint32_t)(((uint64_t)n + TILE - 1) / TILE);
     const uint32_t ld = (T + 3u) & ~3u;
