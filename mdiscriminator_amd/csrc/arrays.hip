@@ -9,16 +9,22 @@
 //
 //   count     word 0 of every element (the key column at stride LEN), per tile
 //   scan      per-class exclusive scan over tiles (shared with the plain path)
-//   partition per tile: (1) flat coalesced loads of the tile's TILE*LEN words,
-//             plus the keys (word 0, the same cache lines); (2) in-wave ballot
-//             rank of the keys, each element's LDS slot recorded in slot_of;
-//             (3) every loaded word is staged at slot*LEN + word in LDS, so
-//             LDS holds the tile's rows grouped by class; (4) LDS word w goes
-//             to out[(dpos[class] + w/LEN)*LEN + w%LEN]: contiguous runs.
+//   partition per tile, rows of 2, 4 or 8 words with vector-aligned buffers
+//             (k_partition_rows_vec): whole rows loaded in (round, lane) order,
+//             ranked by their word 0, staged at their LDS slot and written out
+//             whole.  Other rows (k_partition_rows): (1) flat coalesced loads
+//             of the tile's TILE*LEN words, plus the keys (word 0, the same
+//             cache lines); (2) in-wave ballot rank of the keys, each
+//             element's LDS slot recorded in slot_of; (3) every loaded word is
+//             staged at slot*LEN + word in LDS, so LDS holds the tile's rows
+//             grouped by class; (4) LDS word w goes to
+//             out[(dpos[class] + w/LEN)*LEN + w%LEN]: contiguous runs.
 //
 // HBM bytes per element: 4*LEN (count: the strided key reads fetch the whole
 // rows once LEN*4 < the cache line) + 8*LEN (partition read + write).
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "api_internal.hpp"
 #include "soa_kernels.hpp"
@@ -137,6 +143,95 @@ __global__ __launch_bounds__(NW* kWave) void k_partition_rows(
     }
 }
 
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+// Rows of 2, 4 or 8 words whose input and output are aligned to the row's
+// vector (8 or 16 bytes): moved as whole rows.  Round r of wave w holds rows
+// wbase + r*64 + lane, loaded as one vector per lane (two for LEN 8), so each
+// key (word 0) arrives with its row and needs no second load.  A row is staged
+// at its slot with vector LDS accesses, and slot s goes out whole to output
+// row dpos[class] + s.  Against k_partition_rows this takes one class and
+// offset lookup per row instead of per word.  Same tile as the count pass.
+template <int NW, int R, int LEN, int NBITS, class Discr>
+__global__ __launch_bounds__(NW* kWave) void k_partition_rows_vec(
+    const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t n, Discr dis,
+    const uint32_t* __restrict__ prefix, const uint32_t* __restrict__ totals, uint32_t ld,
+    uint32_t m) {
+    static_assert(LEN == 2 || LEN == 4 || LEN == 8, "rows of whole vectors");
+    using V = typename std::conditional<LEN == 2, u32x2, u32x4>::type;
+    constexpr int VPR = LEN == 8 ? 2 : 1;  // vectors per row
+    constexpr int MC = 1 << NBITS;
+    constexpr int TILE = NW * kWave * R;   // rows
+    constexpr int THREADS = NW * kWave;
+    __shared__ V stage[TILE * VPR];
+    __shared__ uint8_t cls[TILE];          // LDS slot -> class
+    __shared__ uint32_t cursor[NW][MC];
+    __shared__ uint32_t dpos[MC];
+
+    const uint32_t lane = lane_id();
+    const uint32_t wave = wave_id();
+    const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t tbase = t * (uint32_t)TILE;
+    const uint32_t wbase = tbase + wave * (uint32_t)(kWave * R);
+    const bool full = (uint64_t)tbase + TILE <= n;
+    const uint32_t valid = full ? (uint32_t)TILE : n - tbase;
+
+    uint32_t pre = 0, tot = 0;
+    if (wave == 0 && lane < m) {
+        pre = prefix[(size_t)lane * ld + t];
+        tot = totals[lane];
+    }
+
+    // 1. Whole rows in (round, lane) order; word 0 is the key.
+    const V* __restrict__ lp = reinterpret_cast<const V*>(in) + ((size_t)wbase + lane) * VPR;
+    V row[R][VPR];
+    uint32_t v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const bool ok = full || wbase + r * kWave + lane < n;
+#pragma unroll
+        for (int j = 0; j < VPR; ++j)
+            row[r][j] = ok ? __builtin_nontemporal_load(lp + (r * kWave) * VPR + j) : (V)(0u);
+        v[r] = row[r][0].x;
+    }
+    for (int c = lane; c < MC; c += kWave) cursor[wave][c] = 0;
+    __builtin_amdgcn_wave_barrier();
+
+    // 2. In-wave stable rank, then tile offsets (wave 0, lane = class).
+    uint32_t slot[R];
+    if (full) rank_rounds<R, NBITS, true>(v, slot, cursor[wave], wbase, n, m, lane, dis);
+    else rank_rounds<R, NBITS, false>(v, slot, cursor[wave], wbase, n, m, lane, dis);
+    __syncthreads();
+    if (wave == 0) {
+        uint32_t running = wave_excl_scan(tot, lane) + pre;
+        tile_offsets<NW, MC>(cursor, dpos, running, lane);
+    }
+    __syncthreads();
+
+    // 3. Rows into LDS at their slots, grouped by class.
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const uint32_t c = slot[r] >> 16;
+        const uint32_t sl = cursor[wave][c] + (slot[r] & 0xFFFFu);
+        cls[sl] = (uint8_t)c;
+#pragma unroll
+        for (int j = 0; j < VPR; ++j) stage[sl * VPR + j] = row[r][j];
+    }
+    __syncthreads();
+
+    // 4. Slot s to output row dpos[class] + s: contiguous class runs.
+    V* __restrict__ ov = reinterpret_cast<V*>(out);
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const uint32_t sl = k * THREADS + threadIdx.x;
+        if (full || sl < valid) {
+            const size_t d = (size_t)(dpos[cls[sl]] + sl) * VPR;
+#pragma unroll
+            for (int j = 0; j < VPR; ++j) __builtin_nontemporal_store(stage[sl * VPR + j], ov + d + j);
+        }
+    }
+}
+
 template <int LEN, int NBITS, class Discr>
 mdiscr_status launch_rows(mdiscr_ctx* ctx, const uint32_t* in, uint32_t* out, uint32_t n,
                           Discr dis, uint32_t m, uint32_t* user_counts, hipStream_t s,
@@ -159,6 +254,15 @@ mdiscr_status launch_rows(mdiscr_ctx* ctx, const uint32_t* in, uint32_t* out, ui
     launch_tile_scan(counts, prefix, T, ld, m, totals, user_counts, nullptr, s);
     timing_end(ctx, s, tl, "rows_scan");
     timing_begin(ctx, s, tl);
+    constexpr uintptr_t kVecAlign = LEN == 2 ? 8 : 16;
+    if constexpr (LEN == 2 || LEN == 4 || LEN == 8) {
+        if ((((uintptr_t)in | (uintptr_t)out) & (kVecAlign - 1)) == 0) {
+            hipLaunchKernelGGL((k_partition_rows_vec<kNW, R, LEN, NBITS, Discr>), dim3(T),
+                               dim3(kNW * kWave), 0, s, in, out, n, dis, prefix, totals, ld, m);
+            timing_end(ctx, s, tl, "rows_partition");
+            return ctx_hip(ctx, hipGetLastError(), "array partition launch");
+        }
+    }
     hipLaunchKernelGGL((k_partition_rows<kNW, R, LEN, NBITS, Discr>), dim3(T), dim3(kNW * kWave),
                        0, s, in, out, n, dis, prefix, totals, ld, m);
     timing_end(ctx, s, tl, "rows_partition");

@@ -300,3 +300,27 @@ def test_soa_maximum_size(ctx):
                 f"payload class {k}"
     del d_key, d_pay, o_key, o_pay
     torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("length", [2, 4, 8])
+def test_array_elements_word_aligned_only(ctx, length):
+    """Rows whose buffers start 4 bytes past a 16-byte boundary (so rows of 2,
+    4 and 8 words cannot move as whole vectors), input and output each, tail
+    tiles included, against the oracle."""
+    import torch
+    import mdiscriminator_amd as md
+    for n, m in ((4097, 5), (70001, 16)):
+        x = W.uniform_u31(n * length, 900 + length + n % 7)
+        big_in = torch.zeros(n * length + 4, dtype=torch.int32, device="cuda:0")
+        big_in[1:1 + n * length] = torch.from_numpy(x.view(np.int32)).to("cuda:0")
+        d_in = big_in[1:1 + n * length]
+        big_out = torch.full((n * length + 4,), -1, dtype=torch.int32, device="cuda:0")
+        d_out = big_out[1:1 + n * length]
+        d_cnt = torch.empty(m, dtype=torch.int32, device="cuda:0")
+        print(f"arrays word-aligned len={length} n={n} m={m}", flush=True)
+        ctx.partition_arrays(d_in, d_out, length, md.Mod(m), d_counts=d_cnt)
+        torch.cuda.synchronize()
+        ref, ref_cnt = oracle.seq_mdiscr_arrays(x, length, m)
+        assert first_diff(d_out.cpu().numpy().view(np.uint32), ref) < 0
+        assert np.array_equal(d_cnt.cpu().numpy().view(np.uint32), ref_cnt)
+        assert int(big_out[0]) == -1 and int(big_out[1 + n * length]) == -1   # nothing outside
