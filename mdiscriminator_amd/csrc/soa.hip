@@ -20,7 +20,8 @@ namespace {
 // x 8 rounds up to 4 classes; 8 waves x 12 rounds = 6144 elements for 8..32
 // classes (m=16: -1.6%, m=32: -2.1%, select at m=8: -11%; m=4 is 2.9% faster
 // at 8 rounds); 16 waves x 12 rounds = 12288 elements for 64 classes (8
-// rounds 0.416 ms, 12 rounds 0.403 ms, 16 rounds 0.426 ms).  16 rounds at 8
+// rounds 0.416 ms, 12 rounds 0.403 ms, 16 rounds 0.426 ms; 8 waves x 12 or 16
+// rounds 0.465 / 0.497 ms against 0.406 ms in a later A/B).  16 rounds at 8
 // waves loses 20-30% (occupancy).
 constexpr int kCT = 1;    // tiles per partition workgroup
 
@@ -30,10 +31,13 @@ constexpr int kCT = 1;    // tiles per partition workgroup
 #ifndef MDISCR_SOA64_R
 #define MDISCR_SOA64_R 12
 #endif
+#ifndef MDISCR_SOA64_NW
+#define MDISCR_SOA64_NW 16
+#endif
 template <typename ST, int NBITS>
 struct SoaGeom {
     static constexpr bool kLarge = sizeof(ST) == 4 && NBITS >= 6;
-    static constexpr int NW = kLarge ? 16 : 8;
+    static constexpr int NW = kLarge ? MDISCR_SOA64_NW : 8;
     static constexpr int R =
         kLarge ? MDISCR_SOA64_R : (sizeof(ST) == 4 && NBITS >= 3 ? MDISCR_SOA_R : 8);
     static constexpr uint32_t TILE = NW * kWave * R;
