@@ -34,12 +34,25 @@ constexpr int kCT = 1;    // tiles per partition workgroup
 #ifndef MDISCR_SOA64_NW
 #define MDISCR_SOA64_NW 16
 #endif
-template <typename ST, int NBITS>
+#ifndef MDISCR_SOA_NW
+#define MDISCR_SOA_NW 8
+#endif
+// The select class source (three gathered columns, typically 4 + 1 + 1 bytes)
+// at 8..32 classes: 4 waves x 12 rounds = 3072 elements.  Measured on 2^26
+// (uint, uchar, bool) records, m=8, interleaved A/B: count 0.0658 -> 0.0598 ms,
+// partition 0.2862 -> 0.2669 ms against 8 waves (4 x 16: 0.2918 ms).  The same
+// change slows 4-byte key columns (m=16 partition 0.323 -> 0.347 ms).
+#ifndef MDISCR_SEL_NW
+#define MDISCR_SEL_NW 4
+#endif
+template <typename ST, int NBITS, bool SEL = false>
 struct SoaGeom {
     static constexpr bool kLarge = sizeof(ST) == 4 && NBITS >= 6;
-    static constexpr int NW = kLarge ? MDISCR_SOA64_NW : 8;
+    static constexpr bool kMid = sizeof(ST) == 4 && NBITS >= 3 && NBITS < 6;
+    static constexpr int NW =
+        kLarge ? MDISCR_SOA64_NW : (SEL && kMid ? MDISCR_SEL_NW : MDISCR_SOA_NW);
     static constexpr int R =
-        kLarge ? MDISCR_SOA64_R : (sizeof(ST) == 4 && NBITS >= 3 ? MDISCR_SOA_R : 8);
+        kLarge ? MDISCR_SOA64_R : (kMid ? MDISCR_SOA_R : 8);
     static constexpr uint32_t TILE = NW * kWave * R;
     // Tiles per count workgroup (2^26 keys, interleaved A/B, count ms for 1 /
     // 2 / 8 tiles: m=16 0.042 / 0.045 / 0.051, select m=8 0.066 / 0.069 /
@@ -47,23 +60,30 @@ struct SoaGeom {
     static constexpr int CCT = kLarge ? 2 : 1;
 };
 
-// The tile launch_soa uses for these columns and classes (the workspace size).
-uint32_t soa_tile(bool wide, uint32_t nbits) {
+// The tile launch_soa uses for these columns, classes and class source (the
+// workspace size).
+uint32_t soa_tile(bool wide, uint32_t nbits, bool sel) {
     if (wide) return SoaGeom<uint64_t, 0>::TILE;
     if (nbits >= 6) return SoaGeom<uint32_t, 6>::TILE;
-    return nbits >= 3 ? SoaGeom<uint32_t, 3>::TILE : SoaGeom<uint32_t, 0>::TILE;
+    if (nbits >= 3) return sel ? SoaGeom<uint32_t, 3, true>::TILE : SoaGeom<uint32_t, 3>::TILE;
+    return SoaGeom<uint32_t, 0>::TILE;
 }
 static_assert(SoaGeom<uint32_t, 5>::TILE == SoaGeom<uint32_t, 3>::TILE &&
+                  SoaGeom<uint32_t, 5, true>::TILE == SoaGeom<uint32_t, 3, true>::TILE &&
                   SoaGeom<uint32_t, 2>::TILE == SoaGeom<uint32_t, 0>::TILE &&
-                  SoaGeom<uint64_t, 6>::TILE == SoaGeom<uint64_t, 0>::TILE,
+                  SoaGeom<uint32_t, 2, true>::TILE == SoaGeom<uint32_t, 0>::TILE &&
+                  SoaGeom<uint32_t, 6, true>::TILE == SoaGeom<uint32_t, 6>::TILE &&
+                  SoaGeom<uint64_t, 6>::TILE == SoaGeom<uint64_t, 0>::TILE &&
+                  SoaGeom<uint64_t, 3, true>::TILE == SoaGeom<uint64_t, 0>::TILE,
               "soa_tile covers every geometry");
 
 template <int NBITS, typename ST, class Src, class Discr>
 mdiscr_status launch_soa(mdiscr_ctx* ctx, const SoaCols& cols, Src src, uint32_t n, Discr dis,
                          uint32_t m, uint32_t* user_counts, hipStream_t s, uint32_t* ws) {
-    constexpr int NW = SoaGeom<ST, NBITS>::NW, R = SoaGeom<ST, NBITS>::R;
-    constexpr int kCCT = SoaGeom<ST, NBITS>::CCT;
-    constexpr uint32_t TILE = SoaGeom<ST, NBITS>::TILE;
+    using G = SoaGeom<ST, NBITS, std::is_same_v<Src, SelectSrc>>;
+    constexpr int NW = G::NW, R = G::R;
+    constexpr int kCCT = G::CCT;
+    constexpr uint32_t TILE = G::TILE;
     const uint32_t T = (uint32_t)(((uint64_t)n + TILE - 1) / TILE);
     const uint32_t ld = (T + 3u) & ~3u;
     const uint32_t Kc = (T + kCCT - 1) / kCCT;
@@ -173,7 +193,7 @@ mdiscr_status run_soa(mdiscr_ctx* ctx, SoaCols& cols, uint64_t n, const ClassSpe
     }
     bool wide = false;
     for (uint32_t j = 0; j < cols.ncols; ++j) wide |= cols.bytes[j] > 4;
-    const uint64_t tile = soa_tile(wide, class_bits(m));
+    const uint64_t tile = soa_tile(wide, class_bits(m), cs.kind == ClassKind::Select);
     const uint64_t T = (n + tile - 1) / tile;
     const uint64_t ld = (T + 3) & ~3ull;
     void* ws = nullptr;

@@ -277,14 +277,15 @@ __global__ __launch_bounds__(NW* kWave) void k_partition_soa(
             dest[k] = (full || s < valid) ? dpos[cls[s]] + s : 0u;
         }
 
-        // 4. Every column through the staging buffer.
+        // 4. Every column through the staging buffer, the key column first:
+        // its values are already in registers, which are then free for the
+        // other columns' loads.
+        if constexpr (Src::kKeyInRegs)
+            soa_write_keys<typename Src::KeyT, NW, R>(v, cols.out[cols.key], slot, dest, valid,
+                                                      stage);
         for (uint32_t j = 0; j < cols.ncols; ++j) {
             if constexpr (Src::kKeyInRegs) {
-                if (j == cols.key) {
-                    soa_write_keys<typename Src::KeyT, NW, R>(v, cols.out[j], slot, dest, valid,
-                                                              stage);
-                    continue;
-                }
+                if (j == cols.key) continue;
             }
             switch (cols.bytes[j]) {
                 case 1:
