@@ -125,8 +125,14 @@ __global__ __launch_bounds__(THREADS) void k_seg_plan(const SegDesc* __restrict_
     constexpr uint32_t WT = kWave * R;          // one wave's elements
     constexpr uint32_t TILE = NW * kWave * R;   // one workgroup's elements
     constexpr uint32_t SPT = 16;
+    constexpr uint32_t PASS = THREADS * SPT;    // segments per pass
     static_assert(SPT % NW == 0, "whole groups per thread");
     __shared__ uint32_t wsum[THREADS / kWave];
+    // A pass's segment lengths.  Consecutive threads read consecutive
+    // descriptors (every descriptor line fetched once, where one thread per
+    // SPT descriptors touched a line per lane and load), and one word of
+    // padding per SPT entries puts thread t's entries t*SPT+k in distinct banks.
+    __shared__ uint32_t slen[PASS + PASS / SPT];
     // A batch's descriptors arrive in mapped pinned memory: copy them to the
     // device table the later kernels read.
     if (staged) {
@@ -134,11 +140,15 @@ __global__ __launch_bounds__(THREADS) void k_seg_plan(const SegDesc* __restrict_
         __syncthreads();
     }
     uint32_t item_carry = 0, row_carry = 0, long_carry = 0;
-    for (uint32_t base = 0; base < nseg; base += THREADS * SPT) {
+    for (uint32_t base = 0; base < nseg; base += PASS) {
+        const uint32_t cnt = min(nseg - base, PASS);
+        for (uint32_t i = threadIdx.x; i < cnt; i += THREADS) slen[i + i / SPT] = desc[base + i].len;
+        __syncthreads();
         const uint32_t s0 = base + threadIdx.x * SPT;
         uint32_t len[SPT];
 #pragma unroll
-        for (uint32_t k = 0; k < SPT; ++k) len[k] = (s0 + k < nseg) ? desc[s0 + k].len : 0u;
+        for (uint32_t k = 0; k < SPT; ++k)
+            len[k] = (s0 + k < nseg) ? slen[threadIdx.x * (SPT + 1) + k] : 0u;
         uint32_t nitems = 0, nrows = 0, nlong = 0;
 #pragma unroll
         for (uint32_t g = 0; g < SPT; g += NW) {
