@@ -19,6 +19,8 @@
 // Bytes per record: key bytes (count) + 2 * sum(column bytes).
 #pragma once
 
+#include <type_traits>
+
 #include "plain_kernels.hpp"
 
 namespace mdiscr {
@@ -161,13 +163,30 @@ __global__ __launch_bounds__(NW* kWave) void k_src_count(
 }
 
 // ---------------------------------------------------------------------------
+// Where the tile's slot s = k*THREADS + tid goes: dpos[class of s] + s.  Kept
+// in registers (one per slot the thread writes), or, with LDS, looked up again
+// for every column from the slot's class.  The lookup frees R registers
+// (107 -> 85 at 8 waves x 12 rounds) but costs two LDS reads per slot and
+// column: it pays only for the select source (2^26 (uint, uchar, bool)
+// records, m=8: partition 0.267 -> 0.254 ms; 4-byte key columns 2-5% slower).
+template <int R, bool LDS>
+struct SlotDest {
+    uint32_t reg[LDS ? 1 : R];
+    const uint8_t* cls;
+    const uint32_t* dpos;
+    __device__ __forceinline__ uint32_t at(int k, uint32_t s) const {
+        if constexpr (LDS) return dpos[cls[s]] + s;
+        else return reg[k];
+    }
+};
+
 // Moves one column of the current tile: load at the tile's element indices,
-// stage at each element's LDS slot, write slot s to out[dest[s]].
-template <typename T, int NW, int R>
+// stage at each element's LDS slot, write slot s to out[dest(s)].
+template <typename T, int NW, int R, bool DL>
 __device__ __forceinline__ void soa_move_column(const void* __restrict__ src_v,
                                                 void* __restrict__ dst_v, uint32_t wbase,
                                                 uint32_t n, bool full, const uint32_t (&slot)[R],
-                                                const uint32_t (&dest)[R], uint32_t valid,
+                                                const SlotDest<R, DL>& dest, uint32_t valid,
                                                 void* stage_v, uint32_t lane) {
     constexpr int THREADS = NW * kWave;
     // one per-lane pointer; round r is a constant offset
@@ -184,16 +203,16 @@ __device__ __forceinline__ void soa_move_column(const void* __restrict__ src_v,
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         const uint32_t s = k * THREADS + threadIdx.x;
-        if (s < valid) __builtin_nontemporal_store(stage[s], dst + dest[k]);
+        if (s < valid) __builtin_nontemporal_store(stage[s], dst + dest.at(k, s));
     }
     __syncthreads();
 }
 
 // The key column is already in registers: stage and write it without a reload.
-template <typename KT, int NW, int R>
+template <typename KT, int NW, int R, bool DL>
 __device__ __forceinline__ void soa_write_keys(const uint32_t (&v)[R], void* __restrict__ dst_v,
                                                const uint32_t (&slot)[R],
-                                               const uint32_t (&dest)[R], uint32_t valid,
+                                               const SlotDest<R, DL>& dest, uint32_t valid,
                                                void* stage_v) {
     constexpr int THREADS = NW * kWave;
     KT* __restrict__ dst = static_cast<KT*>(dst_v);
@@ -204,7 +223,7 @@ __device__ __forceinline__ void soa_write_keys(const uint32_t (&v)[R], void* __r
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         const uint32_t s = k * THREADS + threadIdx.x;
-        if (s < valid) __builtin_nontemporal_store(stage[s], dst + dest[k]);
+        if (s < valid) __builtin_nontemporal_store(stage[s], dst + dest.at(k, s));
     }
     __syncthreads();
 }
@@ -270,11 +289,16 @@ __global__ __launch_bounds__(NW* kWave) void k_partition_soa(
             cls[slot[r]] = (uint8_t)c;
         }
         __syncthreads();
-        uint32_t dest[R];
+        constexpr bool kDestLds = std::is_same_v<Src, SelectSrc>;
+        SlotDest<R, kDestLds> dest;
+        dest.cls = cls;
+        dest.dpos = dpos;
+        if constexpr (!kDestLds) {
 #pragma unroll
-        for (int k = 0; k < R; ++k) {
-            const uint32_t s = k * THREADS + threadIdx.x;
-            dest[k] = (full || s < valid) ? dpos[cls[s]] + s : 0u;
+            for (int k = 0; k < R; ++k) {
+                const uint32_t s = k * THREADS + threadIdx.x;
+                dest.reg[k] = (full || s < valid) ? dpos[cls[s]] + s : 0u;
+            }
         }
 
         // 4. Every column through the staging buffer, the key column first:
