@@ -41,9 +41,14 @@ constexpr int kCT = 1;    // tiles per partition workgroup
 // at 8..32 classes: 4 waves x 12 rounds = 3072 elements.  Measured on 2^26
 // (uint, uchar, bool) records, m=8, interleaved A/B: count 0.0658 -> 0.0598 ms,
 // partition 0.2862 -> 0.2669 ms against 8 waves (4 x 16: 0.2918 ms).  The same
-// change slows 4-byte key columns (m=16 partition 0.323 -> 0.347 ms).
+// change slows 4-byte key columns (m=16 partition 0.323 -> 0.347 ms).  With the
+// destinations looked up in LDS, whole select calls: 4 x 12 0.320 ms, 4 x 8
+// 0.329, 4 x 16 0.330, 8 x 8 0.337, 2 x 16 0.341.
 #ifndef MDISCR_SEL_NW
 #define MDISCR_SEL_NW 4
+#endif
+#ifndef MDISCR_SEL_R
+#define MDISCR_SEL_R 12
 #endif
 template <typename ST, int NBITS, bool SEL = false>
 struct SoaGeom {
@@ -52,7 +57,7 @@ struct SoaGeom {
     static constexpr int NW =
         kLarge ? MDISCR_SOA64_NW : (SEL && kMid ? MDISCR_SEL_NW : MDISCR_SOA_NW);
     static constexpr int R =
-        kLarge ? MDISCR_SOA64_R : (kMid ? MDISCR_SOA_R : 8);
+        kLarge ? MDISCR_SOA64_R : (kMid ? (SEL ? MDISCR_SEL_R : MDISCR_SOA_R) : 8);
     static constexpr uint32_t TILE = NW * kWave * R;
     // Tiles per count workgroup (2^26 keys, interleaved A/B, count ms for 1 /
     // 2 / 8 tiles: m=16 0.042 / 0.045 / 0.051, select m=8 0.066 / 0.069 /
