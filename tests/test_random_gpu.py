@@ -196,3 +196,73 @@ def test_random_batch(ctx, case):
         ref, ref_cnt = oracle.seq_mdiscr(x, ms[i], kinds[i])
         assert first_diff(host(reqs[i][1], x), ref) < 0, f"request {i}"
         assert np.array_equal(host(reqs[i][3], ref_cnt), ref_cnt), f"request {i}"
+
+
+def rand_record_size(rng):
+    """Sizes near multiples of every SoA tile (3072, 4096, 6144, 12288) or random."""
+    k = int(rng.integers(0, 3))
+    if k == 0:
+        t = int(rng.choice([3072, 4096, 6144, 12288]))
+        return max(0, t * int(rng.integers(1, 40)) + int(rng.integers(-2, 3)))
+    return rand_size(rng)
+
+
+@pytest.mark.parametrize("case", range(CASES))
+def test_random_select(ctx, case):
+    """z ? D(x) : D(y) with random widths (1 or 4 bytes) for the three class
+    columns, which may coincide, plus a random payload column."""
+    import torch
+    import mdiscriminator_amd as md
+    rng = np.random.default_rng(14000 + case)
+    n = rand_record_size(rng)
+    d, kind = rand_discr(rng, md)
+    dt = {1: np.uint8, 2: np.uint16, 4: np.uint32, 8: np.uint64}
+    widths = [int(rng.choice([1, 4])) for _ in range(3)] + [int(rng.choice([1, 2, 4, 8]))]
+    cols = [W.splitmix64(n, 15000 + 10 * case + j).astype(dt[w]) for j, w in enumerate(widths)]
+    sel, a, b = (int(v) for v in rng.integers(0, 4, size=3))
+    while widths[sel] not in (1, 4) or widths[a] not in (1, 4) or widths[b] not in (1, 4):
+        sel, a, b = (int(v) for v in rng.integers(0, 4, size=3))
+    if rng.integers(0, 2):  # a 0/1 selector, as the reference's bool column
+        cols[sel] = (cols[sel] & dt[widths[sel]](1)).astype(dt[widths[sel]])
+    print(f"select case {case}: n={n} m={d.m} kind={kind} widths={widths} sel={sel} a={a} b={b}",
+          flush=True)
+    d_in = [dev(c) for c in cols]
+    d_out = [torch.empty_like(t) for t in d_in]
+    d_cnt = torch.full((d.m,), -1, dtype=torch.int32, device="cuda:0")
+    ctx.partition_soa_select(d_in, d_out, d, sel_col=sel, a_col=a, b_col=b, d_counts=d_cnt)
+    torch.cuda.synchronize()
+    ref, ref_cnt = oracle.seq_mdiscr_soa_select(cols, d.m, sel, a, b, kind=kind)
+    assert np.array_equal(host(d_cnt, ref_cnt), ref_cnt)
+    for j, (o, r) in enumerate(zip(d_out, ref)):
+        assert first_diff(host(o, cols[j]), r) < 0, f"column {j}"
+
+
+@pytest.mark.parametrize("case", range(CASES))
+def test_random_filter_records(ctx, case):
+    """The filter over records: random predicate on a 4-byte key column."""
+    import torch
+    import mdiscriminator_amd as md
+    rng = np.random.default_rng(16000 + case)
+    n = rand_record_size(rng)
+    op = int(rng.integers(0, 8))
+    a, b = int(rng.integers(0, 2**32)), int(rng.integers(0, 2**32))
+    if op == oracle.PRED_BITS:
+        b &= a
+    dt = {1: np.uint8, 2: np.uint16, 4: np.uint32, 8: np.uint64}
+    ncols = int(rng.integers(1, 5))
+    widths = [int(rng.choice([1, 2, 4, 8])) for _ in range(ncols)]
+    key_col = int(rng.integers(0, ncols))
+    widths[key_col] = 4
+    cols = [W.splitmix64(n, 17000 + 10 * case + j).astype(dt[w]) for j, w in enumerate(widths)]
+    cols[key_col] = keys(rng, n, 18000 + case)
+    print(f"filter records case {case}: n={n} op={op} a={a} b={b} widths={widths} key={key_col}",
+          flush=True)
+    d_in = [dev(c) for c in cols]
+    d_out = [torch.empty_like(t) for t in d_in]
+    d_cnt = torch.full((2,), -1, dtype=torch.int32, device="cuda:0")
+    ctx.filter_soa(d_in, d_out, md.Pred(op, a, b), key_col=key_col, d_counts=d_cnt)
+    torch.cuda.synchronize()
+    ref, ref_cnt = oracle.filter_soa(cols, op, a, b, key_col=key_col)
+    assert np.array_equal(host(d_cnt, ref_cnt), ref_cnt)
+    for j, (o, r) in enumerate(zip(d_out, ref)):
+        assert first_diff(host(o, cols[j]), r) < 0, f"column {j}"
