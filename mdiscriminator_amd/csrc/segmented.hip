@@ -353,6 +353,15 @@ mdiscr_status run_segmented(mdiscr_ctx* ctx, const uint32_t* d_in, uint32_t* d_o
     return ctx_hip(ctx, hipGetLastError(), "segmented launch");
 }
 
+// MDISCR_SEG_CONFIG: the configuration every segmented and batch call uses
+// where it is available, or -1.  Read on every call, so a process can change it.
+int forced_config() {
+    const char* f = getenv("MDISCR_SEG_CONFIG");
+    if (!f) return -1;
+    const int c = atoi(f);
+    return c >= 0 && c < kNumConfigs ? c : -1;
+}
+
 int pick_config(mdiscr_ctx* ctx, const uint32_t* d_in, uint32_t* d_out, uint64_t n,
                 const uint64_t* d_seg_off, uint64_t nseg, mdiscr_discr d, hipStream_t s,
                 mdiscr_status* err) {
@@ -360,10 +369,7 @@ int pick_config(mdiscr_ctx* ctx, const uint32_t* d_in, uint32_t* d_out, uint64_t
     const bool mask = d.kind == MDISCR_DISCR_MASK || (d.m & (d.m - 1)) == 0;
     const uint32_t nbits = class_bits(d.m);
     if (!(mask && nbits <= 4)) return 0;
-    if (const char* f = getenv("MDISCR_SEG_CONFIG")) {
-        const int c = atoi(f);
-        if (c >= 0 && c < kNumConfigs && config_available(mask, nbits, c)) return c;
-    }
+    if (const int c = forced_config(); c >= 0 && config_available(mask, nbits, c)) return c;
     const char* at = getenv("MDISCR_AUTOTUNE");
     if (at && at[0] == '0') return 0;
     SegState* ss = seg_state(ctx);
@@ -654,7 +660,10 @@ mdiscr_status mdiscr_partition_batch(mdiscr_ctx* ctx, const mdiscr_request* reqs
     const uint32_t nbits = class_bits(mmax);
     int cfg = 0;
     auto it = ss->tuned.find(bucket_key(all_pow2, nbits, total, nreq));
-    if (it != ss->tuned.end() && config_available(all_pow2, nbits, it->second)) cfg = it->second;
+    if (const int c = forced_config(); c >= 0 && config_available(all_pow2, nbits, c))
+        cfg = c;
+    else if (it != ss->tuned.end() && config_available(all_pow2, nbits, it->second))
+        cfg = it->second;
     SegArgs a;
     a.staged = static_cast<const SegDesc*>(ss->host_desc_dev);  // the planner copies them
     hipError_t staged_err = hipSuccess;
