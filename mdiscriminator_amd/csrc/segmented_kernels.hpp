@@ -234,7 +234,11 @@ __global__ __launch_bounds__(THREADS) void k_seg_expand(const SegLong* __restric
 // tiles are in flight together, which halves the item -> descriptor -> data
 // latency chains per element.  (10k lognormal segments, m=4: the 4 x 8 tile's
 // whole call 0.3153 -> 0.3076 ms; pairs of 4 x 16 tiles were slower, 0.3107 ->
-// 0.3177 ms, so those take one item at a time.)
+// 0.3177 ms, so those take one item at a time.  Three or four items per step:
+// 0.3101-0.3116 ms against 0.3080-0.3083 ms for pairs.)
+#ifndef MDISCR_SEG_COUNT_P
+#define MDISCR_SEG_COUNT_P 2u
+#endif
 template <int NW, int R, int NBITS, bool MASK>
 __global__ __launch_bounds__(NW* kWave) void k_seg_count(const SegDesc* __restrict__ desc,
                                                          const SegItem* __restrict__ items,
@@ -245,7 +249,7 @@ __global__ __launch_bounds__(NW* kWave) void k_seg_count(const SegDesc* __restri
     constexpr int THREADS = NW * kWave;
     constexpr uint32_t TILE = NW * kWave * R;
     constexpr bool PACKED = LaneCounts<NBITS>::kPacked;
-    constexpr uint32_t P = R <= 8 ? 2u : 1u;  // items per iteration
+    constexpr uint32_t P = R <= 8 ? MDISCR_SEG_COUNT_P : 1u;  // items per iteration
     __shared__ uint32_t hist[P][NW][MC];
     const uint32_t lane = lane_id();
     const uint32_t wave = wave_id();
