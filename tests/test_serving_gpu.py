@@ -46,6 +46,36 @@ def test_batch_matches_alone_and_oracle(ctx):
             assert torch.equal(alone, d_out[i])
 
 
+@pytest.mark.parametrize("cfg", list(range(8)))
+def test_batch_every_forced_config(ctx, cfg):
+    """MDISCR_SEG_CONFIG applies to batch calls too: every configuration gives
+    the oracle's result for a power-of-two mix (where all of them are built)
+    with a many-tile request, and falls back to the default elsewhere."""
+    import os
+    import torch
+    import mdiscriminator_amd as md
+    xs, ms = make_requests(40 + cfg, 24, hi=300000)
+    for label, mk in (("pow2", lambda m: md.Mask((2, 4, 8, 16)[m % 4])), ("mod", md.Mod)):
+        discrs = [mk(m) for m in ms]
+        print(f"batch cfg={cfg} {label}", flush=True)
+        d_in = [to_dev(x) if len(x) else torch.empty(0, dtype=torch.int32, device="cuda:0")
+                for x in xs]
+        d_out = [torch.full_like(t, -1) for t in d_in]
+        d_cnt = [torch.full((d.m,), -1, dtype=torch.int32, device="cuda:0") for d in discrs]
+        os.environ["MDISCR_SEG_CONFIG"] = str(cfg)
+        try:
+            ctx.partition_batch([(d_in[i], d_out[i], discrs[i], d_cnt[i]) for i in range(len(xs))])
+            torch.cuda.synchronize()
+        finally:
+            del os.environ["MDISCR_SEG_CONFIG"]
+        for i, x in enumerate(xs):
+            d = discrs[i]
+            ref, ref_cnt = oracle.seq_mdiscr(x, d.m, oracle.MASK if d.kind == md.DISCR_MASK
+                                             else oracle.MOD)
+            assert first_diff(from_dev(d_out[i]), ref) < 0, f"{label} request {i} (n={len(x)})"
+            assert np.array_equal(from_dev(d_cnt[i]), ref_cnt)
+
+
 def test_batch_sequence_on_stream(ctx):
     """Successive batches on one non-null stream (the server's case), reusing
     the context's pinned descriptor staging: other tensors, request counts,
