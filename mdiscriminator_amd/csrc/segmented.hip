@@ -48,6 +48,7 @@ constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
 constexpr uint32_t kMinTile = 1024;  // smallest NW*64*R among the candidates
 
 struct SegArgs {
+    OffSrc os{};                      // segments by offsets (descriptors built on the device)
     const SegDesc* staged = nullptr;  // batch descriptors in mapped pinned memory
     hipEvent_t staged_read = nullptr;  // recorded once the planner has read them
     hipError_t* staged_err = nullptr;  // where that record's status goes
@@ -92,13 +93,13 @@ template <int NW, int R, int NBITS, bool MASK, bool STAGED>
 void launch_cfg(const SegArgs& a, hipStream_t s) {
     const bool expand_here = a.nseg <= kExpandInPlan;
     hipLaunchKernelGGL((k_seg_plan<kPlanThreads, NW, R>), dim3(1), dim3(kPlanThreads), 0, s,
-                       a.staged, const_cast<SegDesc*>(a.desc), a.nseg, a.items, a.longs, a.plan,
-                       a.counts, a.ld, a.mrows, expand_here ? 1u : 0u);
+                       a.staged, a.os, const_cast<SegDesc*>(a.desc), a.nseg, a.items, a.longs,
+                       a.plan, a.counts, a.ld, a.mrows, expand_here ? 1u : 0u);
     if (a.staged_read) *a.staged_err = hipEventRecord(a.staged_read, s);
     if (!expand_here) {
         const uint32_t eg = a.nseg < kExpandGrid ? (a.nseg ? a.nseg : 1u) : kExpandGrid;
         hipLaunchKernelGGL((k_seg_expand<256>), dim3(eg), dim3(256), 0, s, a.longs, a.plan,
-                           a.items);
+                           a.items, a.os, const_cast<SegDesc*>(a.desc), a.nseg);
     }
     static const int count_per_cu = resident_per_cu(k_seg_count<NW, R, NBITS, MASK>, NW * kWave);
     static const int part_per_cu =
@@ -307,12 +308,9 @@ mdiscr_status run_segmented(mdiscr_ctx* ctx, const uint32_t* d_in, uint32_t* d_o
     a.mrows = d.m;
     a.grid_cap = grid_cap(2 * nseg + n / kMinTile + 2);
     a.num_cu = (uint32_t)ctx->num_cu;
-    const mdiscr_divmagic g = mdiscr_divmagic_make(d.m);
+    a.os = OffSrc{d_seg_off, d_in, d_out, d_seg_counts, d.m, mdiscr_divmagic_make(d.m)};
 
     auto issue = [&](hipStream_t q) -> bool {
-        hipLaunchKernelGGL(k_seg_desc_from_offsets, dim3((uint32_t)((nseg + 255) / 256)), dim3(256),
-                           0, q, d_seg_off, (uint32_t)nseg, d_in, d_out, d_seg_counts, d.m, g,
-                           w.desc);
         return mask ? launch_any<true>(nbits, cfg, a, q) : launch_any<false>(nbits, cfg, a, q);
     };
 

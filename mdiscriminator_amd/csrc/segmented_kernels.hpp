@@ -68,22 +68,28 @@ struct SegDiscr<false> {
 };
 
 // ---------------------------------------------------------------------------
-// One descriptor per segment of a single array given by offsets.
-__global__ void k_seg_desc_from_offsets(const uint64_t* __restrict__ off, uint32_t nseg,
-                                        const uint32_t* in, uint32_t* out,
-                                        uint32_t* seg_counts, uint32_t m, mdiscr_divmagic g,
-                                        SegDesc* __restrict__ desc) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nseg) return;
-    const uint64_t a = off[s], b = off[s + 1];
+// Segments of one array given by offsets: what every descriptor is built
+// from.  With off null, the descriptors are already in the device table (a
+// batch's, copied there by the planner).
+struct OffSrc {
+    const uint64_t* off;   // nseg + 1 offsets, or null
+    const uint32_t* in;
+    uint32_t* out;
+    uint32_t* counts;      // m per segment, or null
+    uint32_t m;
+    mdiscr_divmagic g;
+};
+
+__device__ __forceinline__ SegDesc desc_from_offsets(const OffSrc& os, uint32_t s) {
+    const uint64_t a = os.off[s], b = os.off[s + 1];
     SegDesc d;
-    d.in = in + a;
-    d.out = out + a;
-    d.counts = seg_counts ? seg_counts + (size_t)s * m : nullptr;
+    d.in = os.in + a;
+    d.out = os.out + a;
+    d.counts = os.counts ? os.counts + (size_t)s * os.m : nullptr;
     d.len = (uint32_t)(b - a);
-    d.m = m;
-    d.g = g;
-    desc[s] = d;
+    d.m = os.m;
+    d.g = os.g;
+    return d;
 }
 
 // ---------------------------------------------------------------------------
@@ -109,6 +115,7 @@ __device__ __forceinline__ uint32_t block_excl_sum(uint32_t v, uint32_t* wsum, u
 
 template <int THREADS, int NW, int R>
 __global__ __launch_bounds__(THREADS) void k_seg_plan(const SegDesc* __restrict__ staged,
+                                                      OffSrc os,
                                                       SegDesc* __restrict__ desc, uint32_t nseg,
                                                       SegItem* __restrict__ items,
                                                       SegLong* __restrict__ longs,
@@ -142,7 +149,23 @@ __global__ __launch_bounds__(THREADS) void k_seg_plan(const SegDesc* __restrict_
     uint32_t item_carry = 0, row_carry = 0, long_carry = 0;
     for (uint32_t base = 0; base < nseg; base += PASS) {
         const uint32_t cnt = min(nseg - base, PASS);
-        for (uint32_t i = threadIdx.x; i < cnt; i += THREADS) slen[i + i / SPT] = desc[base + i].len;
+        if (os.off) {
+            // Lengths from the offsets (consecutive 8-byte loads).  Few
+            // segments: the planner also writes their descriptors, since no
+            // expansion kernel runs.
+            for (uint32_t i = threadIdx.x; i < cnt; i += THREADS) {
+                if (expand_here) {
+                    const SegDesc d = desc_from_offsets(os, base + i);
+                    desc[base + i] = d;
+                    slen[i + i / SPT] = d.len;
+                } else {
+                    slen[i + i / SPT] = (uint32_t)(os.off[base + i + 1] - os.off[base + i]);
+                }
+            }
+        } else {
+            for (uint32_t i = threadIdx.x; i < cnt; i += THREADS)
+                slen[i + i / SPT] = desc[base + i].len;
+        }
         __syncthreads();
         const uint32_t s0 = base + threadIdx.x * SPT;
         uint32_t len[SPT];
@@ -217,7 +240,13 @@ __global__ __launch_bounds__(THREADS) void k_seg_plan(const SegDesc* __restrict_
 template <int THREADS>
 __global__ __launch_bounds__(THREADS) void k_seg_expand(const SegLong* __restrict__ longs,
                                                         const SegPlan* __restrict__ plan,
-                                                        SegItem* __restrict__ items) {
+                                                        SegItem* __restrict__ items, OffSrc os,
+                                                        SegDesc* __restrict__ desc,
+                                                        uint32_t nseg) {
+    // Segments given by offsets: every descriptor, grid-stride.
+    if (os.off)
+        for (uint32_t i = blockIdx.x * THREADS + threadIdx.x; i < nseg; i += gridDim.x * THREADS)
+            desc[i] = desc_from_offsets(os, i);
     const uint32_t n_long = plan->n_long;
     for (uint32_t k = blockIdx.x; k < n_long; k += gridDim.x) {
         const SegLong L = longs[k];
