@@ -136,9 +136,9 @@ __global__ __launch_bounds__(THREADS) void k_seg_plan(const SegDesc* __restrict_
     static_assert(SPT % NW == 0, "whole groups per thread");
     __shared__ uint32_t wsum[THREADS / kWave];
     // A pass's segment lengths.  Consecutive threads read consecutive
-    // descriptors (every descriptor line fetched once, where one thread per
-    // SPT descriptors touched a line per lane and load), and one word of
-    // padding per SPT entries puts thread t's entries t*SPT+k in distinct banks.
+    // offsets or descriptors (every line fetched once, where one thread per
+    // SPT segments touched a line per lane and load), and one word of padding
+    // per SPT entries puts thread t's entries t*SPT+k in distinct banks.
     __shared__ uint32_t slen[PASS + PASS / SPT];
     // A batch's descriptors arrive in mapped pinned memory: copy them to the
     // device table the later kernels read.
@@ -236,7 +236,8 @@ __global__ __launch_bounds__(THREADS) void k_seg_plan(const SegDesc* __restrict_
     }
 }
 
-// The T items of the multi-tile segments: one workgroup per SegLong record.
+// The T items of the multi-tile segments (one workgroup per SegLong record)
+// and, for segments given by offsets, every descriptor.
 template <int THREADS>
 __global__ __launch_bounds__(THREADS) void k_seg_expand(const SegLong* __restrict__ longs,
                                                         const SegPlan* __restrict__ plan,
