@@ -96,21 +96,31 @@ __device__ __forceinline__ SegDesc desc_from_offsets(const OffSrc& os, uint32_t 
 // Planning: one workgroup of THREADS walks the segments in passes, with
 // block scans for the item and row counts.
 template <int THREADS>
-__device__ __forceinline__ uint32_t block_excl_sum(uint32_t v, uint32_t* wsum, uint32_t& total) {
+__device__ __forceinline__ void block_excl_sum3(const uint32_t (&v)[3], uint32_t (*wsum)[3],
+                                                uint32_t (&pre)[3], uint32_t (&total)[3]) {
+    // Three exclusive block sums sharing one pair of barriers (small batch
+    // calls 27.6 -> 27.1 us against one block sum, and its barriers, each).
     constexpr int NWAVE = THREADS / kWave;
     const uint32_t lane = lane_id(), wave = threadIdx.x / kWave;
-    const uint32_t wex = wave_excl_scan(v, lane);
-    if (lane == kWave - 1) wsum[wave] = wex + v;
-    __syncthreads();
-    uint32_t pre = 0, tot = 0;
+    uint32_t wex[3];
 #pragma unroll
-    for (int w = 0; w < NWAVE; ++w) {
-        pre += (w < (int)wave) ? wsum[w] : 0u;
-        tot += wsum[w];
+    for (int q = 0; q < 3; ++q) {
+        wex[q] = wave_excl_scan(v[q], lane);
+        if (lane == kWave - 1) wsum[wave][q] = wex[q] + v[q];
     }
     __syncthreads();
-    total = tot;
-    return pre + wex;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        uint32_t p = 0, t = 0;
+#pragma unroll
+        for (int w = 0; w < NWAVE; ++w) {
+            p += (w < (int)wave) ? wsum[w][q] : 0u;
+            t += wsum[w][q];
+        }
+        pre[q] = p + wex[q];
+        total[q] = t;
+    }
+    __syncthreads();
 }
 
 template <int THREADS, int NW, int R>
@@ -134,7 +144,7 @@ __global__ __launch_bounds__(THREADS) void k_seg_plan(const SegDesc* __restrict_
     constexpr uint32_t SPT = 16;
     constexpr uint32_t PASS = THREADS * SPT;    // segments per pass
     static_assert(SPT % NW == 0, "whole groups per thread");
-    __shared__ uint32_t wsum[THREADS / kWave];
+    __shared__ uint32_t wsum[THREADS / kWave][3];
     // A pass's segment lengths.  Consecutive threads read consecutive
     // offsets or descriptors (every line fetched once, where one thread per
     // SPT segments touched a line per lane and load), and one word of padding
@@ -191,10 +201,13 @@ __global__ __launch_bounds__(THREADS) void k_seg_plan(const SegDesc* __restrict_
             }
             nitems += any_short ? 1u : 0u;
         }
-        uint32_t itot, rtot, ltot;
-        uint32_t ib = item_carry + block_excl_sum<THREADS>(nitems, wsum, itot);
-        uint32_t rb = row_carry + block_excl_sum<THREADS>(nrows, wsum, rtot);
-        uint32_t lb = long_carry + block_excl_sum<THREADS>(nlong, wsum, ltot);
+        const uint32_t part[3] = {nitems, nrows, nlong};
+        uint32_t pre[3], tot[3];
+        block_excl_sum3<THREADS>(part, wsum, pre, tot);
+        uint32_t ib = item_carry + pre[0];
+        uint32_t rb = row_carry + pre[1];
+        uint32_t lb = long_carry + pre[2];
+        const uint32_t itot = tot[0], rtot = tot[1], ltot = tot[2];
 #pragma unroll
         for (uint32_t g = 0; g < SPT; g += NW) {
             uint32_t mask = 0;
