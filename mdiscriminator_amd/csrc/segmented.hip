@@ -27,7 +27,7 @@ namespace {
 
 constexpr int kPlanThreads = 1024;
 constexpr uint32_t kExpandGrid = 1024;  // workgroups writing the T items of long segments
-constexpr uint32_t kExpandInPlan = 256;  // up to this many segments the planner expands them
+constexpr uint32_t kExpandInPlan = kPlanExpandMax;  // the planner expands them itself
 
 struct SegConfig {
     int nw, r;

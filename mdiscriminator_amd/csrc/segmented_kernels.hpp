@@ -56,6 +56,10 @@ struct SegLong {
     uint32_t row;
 };
 
+// Up to this many segments the planner writes the T items of the multi-tile
+// ones itself (one serving batch), and no expansion kernel runs.
+constexpr uint32_t kPlanExpandMax = 256;
+
 template <bool MASK>
 struct SegDiscr;
 template <>
@@ -150,6 +154,11 @@ __global__ __launch_bounds__(THREADS) void k_seg_plan(const SegDesc* __restrict_
     // SPT segments touched a line per lane and load), and one word of padding
     // per SPT entries puts thread t's entries t*SPT+k in distinct banks.
     __shared__ uint32_t slen[PASS + PASS / SPT];
+    // Few segments (expand_here, at most kPlanExpandMax): the multi-tile
+    // records stay in LDS for the expansion at the end, which then reads them
+    // without one dependent global load per segment (batch of 16 requests,
+    // 23M elements: 0.0890 -> 0.0873 ms).
+    __shared__ SegLong slong[kPlanExpandMax];
     // A batch's descriptors arrive in mapped pinned memory: copy them to the
     // device table the later kernels read.
     if (staged) {
@@ -223,7 +232,9 @@ __global__ __launch_bounds__(THREADS) void k_seg_plan(const SegDesc* __restrict_
                 if (tiles == 1) {
                     items[ib] = SegItem{s0 + k, 0u, 1u, 0u};
                 } else {
-                    longs[lb++] = SegLong{s0 + k, ib, tiles, rb};
+                    const SegLong L{s0 + k, ib, tiles, rb};
+                    if (expand_here) slong[lb++] = L;
+                    else longs[lb++] = L;
                     rb += tiles;
                 }
                 ib += tiles;
@@ -242,7 +253,7 @@ __global__ __launch_bounds__(THREADS) void k_seg_plan(const SegDesc* __restrict_
     if (expand_here) {
         __syncthreads();
         for (uint32_t k = 0; k < long_carry; ++k) {
-            const SegLong L = longs[k];
+            const SegLong L = slong[k];
             for (uint32_t j = threadIdx.x; j < L.tiles; j += THREADS)
                 items[L.item + j] = SegItem{L.seg, j, L.tiles, L.row + j};
         }
