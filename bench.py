@@ -145,6 +145,9 @@ def main():
                     help="for rocprofv3 counter runs: a few partition steps, then an "
                          "elementwise copy of the same 2^28 uint32 (known 1 GiB read + "
                          "1 GiB write) that calibrates FETCH_SIZE / WRITE_SIZE")
+    ap.add_argument("--profile-records", action="store_true",
+                    help="with --profile-mode: also a few SoA partitions of 2^26 "
+                         "(key, f32, index) records at m = 16, 32, 64")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -176,6 +179,17 @@ def main():
         for _ in range(args.steps):
             step()
         torch.cuda.synchronize()
+        if args.profile_records:
+            ns = 1 << 26
+            cols = [W.uniform_u31_torch(ns, SEED + 1, "cuda:0"),
+                    torch.randn(ns, device="cuda:0").view(torch.int32),
+                    torch.arange(ns, dtype=torch.int32, device="cuda:0")]
+            outs = [torch.empty_like(c) for c in cols]
+            for m_ in (16, 32, 64):
+                for _ in range(args.steps):
+                    ctx.partition_soa(cols, outs, md.Mask(m_), stream=stream)
+            torch.cuda.synchronize()
+            del cols, outs
         f_in, f_out = d_in.view(torch.float32), d_out.view(torch.float32)
         for _ in range(args.steps):
             torch.neg(f_in, out=f_out)   # 1 GiB read + 1 GiB write, counter calibration
