@@ -239,6 +239,17 @@ mdiscr_status launch_plain(mdiscr_ctx* ctx, const uint32_t* in, uint32_t* out, u
     uint32_t* totals = ws + (size_t)2 * m * ld; // m
     const bool vec = ((uintptr_t)in & 15u) == 0;
     TimedLaunch tl;
+    if (T == 1) {
+        // One tile: the partition counts it itself (3 launches -> 1; a call on
+        // 4096 elements or fewer is bound by launch latency, not bandwidth).
+        timing_begin(ctx, s, tl);
+        hipLaunchKernelGGL((k_partition<kNW, kR, kCT, NBITS, kPrefetch, MDISCR_PLAIN_WB,
+                                        MDISCR_PLAIN_MINW, Discr, true>),
+                           dim3(1), dim3(kNW * kWave), 0, s, in, out, n, dis, nullptr, nullptr,
+                           0u, m, user_counts);
+        timing_end(ctx, s, tl, "partition");
+        return ctx_hip(ctx, hipGetLastError(), "plain partition launch");
+    }
     timing_begin(ctx, s, tl);
     if (vec)
         hipLaunchKernelGGL((k_chunk_count<kNW, kR, kCCT, NBITS, true, Discr>), dim3(Kc),
@@ -252,7 +263,8 @@ mdiscr_status launch_plain(mdiscr_ctx* ctx, const uint32_t* in, uint32_t* out, u
     timing_end(ctx, s, tl, "tile_scan");
     timing_begin(ctx, s, tl);
     hipLaunchKernelGGL((k_partition<kNW, kR, kCT, NBITS, kPrefetch, MDISCR_PLAIN_WB, MDISCR_PLAIN_MINW, Discr>),
-                       dim3(Kp), dim3(kNW * kWave), 0, s, in, out, n, dis, prefix, totals, ld, m);
+                       dim3(Kp), dim3(kNW * kWave), 0, s, in, out, n, dis, prefix, totals, ld, m,
+                       nullptr);
     timing_end(ctx, s, tl, "partition");
     return ctx_hip(ctx, hipGetLastError(), "plain partition launch");
 }

@@ -452,11 +452,17 @@ __device__ __forceinline__ void write_runs(const uint32_t* stage, const uint32_t
 // ---------------------------------------------------------------------------
 // partition: each workgroup runs its chunk's tiles in order, carrying the
 // running global offset of every class across tiles (wave 0, lane c).
-template <int NW, int R, int CT, int NBITS, bool PREFETCH, int WB, int MINW, class Discr>
+//
+// SINGLE: the input is one tile (n <= TILE), launched as one workgroup with no
+// count or scan pass before it: the tile's own class counts are the totals,
+// and the kernel writes them to user_counts (when given).
+template <int NW, int R, int CT, int NBITS, bool PREFETCH, int WB, int MINW, class Discr,
+          bool SINGLE = false>
 __global__ __launch_bounds__(NW* kWave, MINW) void k_partition(
     const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t n,
     Discr dis, const uint32_t* __restrict__ prefix,
-    const uint32_t* __restrict__ totals, uint32_t ld, uint32_t m) {
+    const uint32_t* __restrict__ totals, uint32_t ld, uint32_t m,
+    uint32_t* __restrict__ user_counts) {
     using G = Geom<NW, R, CT>;
     constexpr int MC = 1 << NBITS;
     constexpr int TILE = G::kTile;
@@ -480,7 +486,7 @@ __global__ __launch_bounds__(NW* kWave, MINW) void k_partition(
     // the running position of the next class-c element is formed at the first
     // tile's phase 2, after these loads have landed.
     uint32_t pre = 0, tot = 0;
-    if (wave == 0 && lane < m) {
+    if (!SINGLE && wave == 0 && lane < m) {
         pre = prefix[(size_t)lane * ld + t0];
         tot = totals[lane];
     }
@@ -518,7 +524,18 @@ __global__ __launch_bounds__(NW* kWave, MINW) void k_partition(
 
         // 2. Per-class LDS cursors and global offsets.
         if (wave == 0) {
-            if (t == t0) running = wave_excl_scan(tot, lane) + pre;
+            if constexpr (SINGLE) {
+                // Elements past n were ranked as class m-1: not counted.
+                uint32_t ct = 0;
+                if (lane < (uint32_t)MC)
+#pragma unroll
+                    for (int w = 0; w < NW; ++w) ct += cursor[w][lane];
+                running = wave_excl_scan(ct, lane);
+                if (user_counts && lane < m)
+                    user_counts[lane] = ct - (lane == m - 1u ? (uint32_t)TILE - n : 0u);
+            } else if (t == t0) {
+                running = wave_excl_scan(tot, lane) + pre;
+            }
             tile_offsets<NW, MC>(cursor, dpos, running, lane);
         }
         __syncthreads();

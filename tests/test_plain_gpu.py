@@ -115,6 +115,30 @@ def test_reference_shaped_call(ctx):
     assert np.array_equal(from_dev(d_out2), ref)
 
 
+@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 1000, 4095, 4096, 4097])
+def test_one_tile_inputs(ctx, n):
+    """Inputs of at most one tile take a single launch that counts its own
+    tile; the counts come back through both the synchronous reference-shaped
+    call and the stream-ordered one, for every class count up to 64."""
+    import torch
+    import mdiscriminator_amd as md
+    for m in (1, 2, 3, 5, 8, 13, 16, 33, 64):
+        x = W.uniform_u31(n, W.SEED_BASE + 7 * n + m)
+        d_in = to_dev(x)
+        for d, kind in ((md.Mod(m), oracle.MOD), (md.Mask(1 << (m.bit_length() - 1)), oracle.MASK)):
+            print(f"one tile n={n} m={d.m} kind={kind}", flush=True)
+            ref, ref_cnt = oracle.seq_mdiscr(x, d.m, kind)
+            d_out = torch.full_like(d_in, -1)
+            assert ctx.mdiscr(d, n, 1 << 16, d_in, d_out) == ref_cnt.tolist()
+            assert np.array_equal(from_dev(d_out), ref)
+            d_out = torch.full_like(d_in, -1)
+            d_cnt = torch.full((d.m,), -1, dtype=torch.int32, device="cuda:0")
+            ctx.partition_u32(d_in, d_out, d, d_counts=d_cnt)
+            torch.cuda.synchronize()
+            assert np.array_equal(from_dev(d_out), ref)
+            assert np.array_equal(from_dev(d_cnt), ref_cnt)
+
+
 def test_sizes_from_counts_device(ctx):
     import torch
     for counts in ([3, 0, 0, 3], [0, 5, 0, 2], [4, 4, 0, 0], [1], [0, 0, 9]):
