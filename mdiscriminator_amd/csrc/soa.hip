@@ -97,6 +97,15 @@ mdiscr_status launch_soa(mdiscr_ctx* ctx, const SoaCols& cols, Src src, uint32_t
     uint32_t* prefix = ws + (size_t)m * ld;
     uint32_t* totals = ws + (size_t)2 * m * ld;
     TimedLaunch tl;
+    if (T == 1) {
+        // One tile: the partition counts it itself (one launch instead of three).
+        timing_begin(ctx, s, tl);
+        hipLaunchKernelGGL((k_partition_soa<NW, R, kCT, NBITS, Src, ST, Discr, true>), dim3(1),
+                           dim3(NW * kWave), 0, s, cols, src, n, dis, nullptr, nullptr, 0u, m,
+                           user_counts);
+        timing_end(ctx, s, tl, "soa_partition");
+        return ctx_hip(ctx, hipGetLastError(), "SoA partition launch");
+    }
     timing_begin(ctx, s, tl);
     bool vec = false;
     if constexpr (std::is_same_v<Src, KeySrc<uint32_t>>)
@@ -115,7 +124,7 @@ mdiscr_status launch_soa(mdiscr_ctx* ctx, const SoaCols& cols, Src src, uint32_t
     timing_end(ctx, s, tl, "soa_scan");
     timing_begin(ctx, s, tl);
     hipLaunchKernelGGL((k_partition_soa<NW, R, kCT, NBITS, Src, ST, Discr>), dim3(Kp),
-                       dim3(NW * kWave), 0, s, cols, src, n, dis, prefix, totals, ld, m);
+                       dim3(NW * kWave), 0, s, cols, src, n, dis, prefix, totals, ld, m, nullptr);
     timing_end(ctx, s, tl, "soa_partition");
     return ctx_hip(ctx, hipGetLastError(), "SoA partition launch");
 }

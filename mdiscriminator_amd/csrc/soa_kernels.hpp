@@ -228,10 +228,15 @@ __device__ __forceinline__ void soa_write_keys(const uint32_t (&v)[R], void* __r
     __syncthreads();
 }
 
-template <int NW, int R, int CT, int NBITS, class Src, typename ST, class Discr>
+// SINGLE: the records are one tile, launched as one workgroup with no count or
+// scan pass: the tile's own class counts are the totals, written to
+// user_counts (when given).
+template <int NW, int R, int CT, int NBITS, class Src, typename ST, class Discr,
+          bool SINGLE = false>
 __global__ __launch_bounds__(NW* kWave) void k_partition_soa(
     SoaCols cols, Src src, uint32_t n, Discr dis, const uint32_t* __restrict__ prefix,
-    const uint32_t* __restrict__ totals, uint32_t ld, uint32_t m) {
+    const uint32_t* __restrict__ totals, uint32_t ld, uint32_t m,
+    uint32_t* __restrict__ user_counts) {
     constexpr int MC = 1 << NBITS;
     constexpr int TILE = NW * kWave * R;
     constexpr int THREADS = NW * kWave;
@@ -248,7 +253,7 @@ __global__ __launch_bounds__(NW* kWave) void k_partition_soa(
     const uint32_t t1 = min(t0 + (uint32_t)CT, T);
 
     uint32_t pre = 0, tot = 0;
-    if (wave == 0 && lane < m) {
+    if (!SINGLE && wave == 0 && lane < m) {
         pre = prefix[(size_t)lane * ld + t0];
         tot = totals[lane];
     }
@@ -276,7 +281,18 @@ __global__ __launch_bounds__(NW* kWave) void k_partition_soa(
 
         // 2. LDS cursors and global offsets (wave 0, lane = class).
         if (wave == 0) {
-            if (t == t0) running = wave_excl_scan(tot, lane) + pre;
+            if constexpr (SINGLE) {
+                // Records past n were ranked as class m-1: not counted.
+                uint32_t ct = 0;
+                if (lane < (uint32_t)MC)
+#pragma unroll
+                    for (int w = 0; w < NW; ++w) ct += cursor[w][lane];
+                running = wave_excl_scan(ct, lane);
+                if (user_counts && lane < m)
+                    user_counts[lane] = ct - (lane == m - 1u ? (uint32_t)TILE - n : 0u);
+            } else if (t == t0) {
+                running = wave_excl_scan(tot, lane) + pre;
+            }
             tile_offsets<NW, MC>(cursor, dpos, running, lane);
         }
         __syncthreads();
