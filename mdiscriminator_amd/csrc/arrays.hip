@@ -52,11 +52,11 @@ constexpr int rounds_for(int len) {
     return MDISCR_ROWS_WPT / len < 2 ? 2 : MDISCR_ROWS_WPT / len;
 }
 
-template <int NW, int R, int LEN, int NBITS, class Discr>
+template <int NW, int R, int LEN, int NBITS, class Discr, bool SINGLE = false>
 __global__ __launch_bounds__(NW* kWave) void k_partition_rows(
     const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t n, Discr dis,
     const uint32_t* __restrict__ prefix, const uint32_t* __restrict__ totals, uint32_t ld,
-    uint32_t m) {
+    uint32_t m, uint32_t* __restrict__ user_counts) {
     constexpr int MC = 1 << NBITS;
     constexpr int TILE = NW * kWave * R;  // elements
     constexpr int THREADS = NW * kWave;
@@ -78,7 +78,7 @@ __global__ __launch_bounds__(NW* kWave) void k_partition_rows(
     const uint32_t* __restrict__ tin = in + (size_t)tbase * LEN;
 
     uint32_t pre = 0, tot = 0;
-    if (wave == 0 && lane < m) {
+    if (!SINGLE && wave == 0 && lane < m) {
         pre = prefix[(size_t)lane * ld + t];
         tot = totals[lane];
     }
@@ -109,7 +109,20 @@ __global__ __launch_bounds__(NW* kWave) void k_partition_rows(
     else rank_rounds<R, NBITS, false>(v, slot, cursor[wave], wbase, n, m, lane, dis);
     __syncthreads();
     if (wave == 0) {
-        uint32_t running = wave_excl_scan(tot, lane) + pre;
+        uint32_t running;
+        if constexpr (SINGLE) {
+            // One tile is all the rows: its class counts are the totals.
+            // Rows past n were ranked as class m-1: not counted.
+            uint32_t ct = 0;
+            if (lane < (uint32_t)MC)
+#pragma unroll
+                for (int w = 0; w < NW; ++w) ct += cursor[w][lane];
+            running = wave_excl_scan(ct, lane);
+            if (user_counts && lane < m)
+                user_counts[lane] = ct - (lane == m - 1u ? (uint32_t)TILE - n : 0u);
+        } else {
+            running = wave_excl_scan(tot, lane) + pre;
+        }
         tile_offsets<NW, MC>(cursor, dpos, running, lane);
     }
     __syncthreads();
@@ -152,11 +165,11 @@ typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 // at its slot with vector LDS accesses, and slot s goes out whole to output
 // row dpos[class] + s.  Against k_partition_rows this takes one class and
 // offset lookup per row instead of per word.  Same tile as the count pass.
-template <int NW, int R, int LEN, int NBITS, class Discr>
+template <int NW, int R, int LEN, int NBITS, class Discr, bool SINGLE = false>
 __global__ __launch_bounds__(NW* kWave) void k_partition_rows_vec(
     const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t n, Discr dis,
     const uint32_t* __restrict__ prefix, const uint32_t* __restrict__ totals, uint32_t ld,
-    uint32_t m) {
+    uint32_t m, uint32_t* __restrict__ user_counts) {
     static_assert(LEN == 2 || LEN == 4 || LEN == 8, "rows of whole vectors");
     using V = typename std::conditional<LEN == 2, u32x2, u32x4>::type;
     constexpr int VPR = LEN == 8 ? 2 : 1;  // vectors per row
@@ -177,7 +190,7 @@ __global__ __launch_bounds__(NW* kWave) void k_partition_rows_vec(
     const uint32_t valid = full ? (uint32_t)TILE : n - tbase;
 
     uint32_t pre = 0, tot = 0;
-    if (wave == 0 && lane < m) {
+    if (!SINGLE && wave == 0 && lane < m) {
         pre = prefix[(size_t)lane * ld + t];
         tot = totals[lane];
     }
@@ -203,7 +216,20 @@ __global__ __launch_bounds__(NW* kWave) void k_partition_rows_vec(
     else rank_rounds<R, NBITS, false>(v, slot, cursor[wave], wbase, n, m, lane, dis);
     __syncthreads();
     if (wave == 0) {
-        uint32_t running = wave_excl_scan(tot, lane) + pre;
+        uint32_t running;
+        if constexpr (SINGLE) {
+            // One tile is all the rows: its class counts are the totals.
+            // Rows past n were ranked as class m-1: not counted.
+            uint32_t ct = 0;
+            if (lane < (uint32_t)MC)
+#pragma unroll
+                for (int w = 0; w < NW; ++w) ct += cursor[w][lane];
+            running = wave_excl_scan(ct, lane);
+            if (user_counts && lane < m)
+                user_counts[lane] = ct - (lane == m - 1u ? (uint32_t)TILE - n : 0u);
+        } else {
+            running = wave_excl_scan(tot, lane) + pre;
+        }
         tile_offsets<NW, MC>(cursor, dpos, running, lane);
     }
     __syncthreads();
@@ -245,6 +271,26 @@ mdiscr_status launch_rows(mdiscr_ctx* ctx, const uint32_t* in, uint32_t* out, ui
     uint32_t* prefix = ws + (size_t)m * ld;
     uint32_t* totals = ws + (size_t)2 * m * ld;
     TimedLaunch tl;
+    constexpr uintptr_t kVecAlign = LEN == 2 ? 8 : 16;
+    if (T == 1) {
+        // One tile: the partition counts it itself (one launch instead of three).
+        timing_begin(ctx, s, tl);
+        bool done = false;
+        if constexpr (LEN == 2 || LEN == 4 || LEN == 8) {
+            if ((((uintptr_t)in | (uintptr_t)out) & (kVecAlign - 1)) == 0) {
+                hipLaunchKernelGGL((k_partition_rows_vec<kNW, R, LEN, NBITS, Discr, true>), dim3(1),
+                                   dim3(kNW * kWave), 0, s, in, out, n, dis, nullptr, nullptr,
+                                   0u, m, user_counts);
+                done = true;
+            }
+        }
+        if (!done)
+            hipLaunchKernelGGL((k_partition_rows<kNW, R, LEN, NBITS, Discr, true>), dim3(1),
+                               dim3(kNW * kWave), 0, s, in, out, n, dis, nullptr, nullptr, 0u, m,
+                               user_counts);
+        timing_end(ctx, s, tl, "rows_partition");
+        return ctx_hip(ctx, hipGetLastError(), "array partition launch");
+    }
     timing_begin(ctx, s, tl);
     hipLaunchKernelGGL((k_src_count<kNW, R, kCCT, NBITS, KeySrc<uint32_t>, Discr>),
                        dim3((T + kCCT - 1) / kCCT), dim3(kNW * kWave), 0, s,
@@ -254,17 +300,17 @@ mdiscr_status launch_rows(mdiscr_ctx* ctx, const uint32_t* in, uint32_t* out, ui
     launch_tile_scan(counts, prefix, T, ld, m, totals, user_counts, nullptr, s);
     timing_end(ctx, s, tl, "rows_scan");
     timing_begin(ctx, s, tl);
-    constexpr uintptr_t kVecAlign = LEN == 2 ? 8 : 16;
     if constexpr (LEN == 2 || LEN == 4 || LEN == 8) {
         if ((((uintptr_t)in | (uintptr_t)out) & (kVecAlign - 1)) == 0) {
             hipLaunchKernelGGL((k_partition_rows_vec<kNW, R, LEN, NBITS, Discr>), dim3(T),
-                               dim3(kNW * kWave), 0, s, in, out, n, dis, prefix, totals, ld, m);
+                               dim3(kNW * kWave), 0, s, in, out, n, dis, prefix, totals, ld, m,
+                               nullptr);
             timing_end(ctx, s, tl, "rows_partition");
             return ctx_hip(ctx, hipGetLastError(), "array partition launch");
         }
     }
     hipLaunchKernelGGL((k_partition_rows<kNW, R, LEN, NBITS, Discr>), dim3(T), dim3(kNW * kWave),
-                       0, s, in, out, n, dis, prefix, totals, ld, m);
+                       0, s, in, out, n, dis, prefix, totals, ld, m, nullptr);
     timing_end(ctx, s, tl, "rows_partition");
     return ctx_hip(ctx, hipGetLastError(), "array partition launch");
 }
