@@ -9,7 +9,8 @@ into m = 8 classes (class = x & 7), inputs resident in HBM, through the C ABI
   roofline      the partition kernel's algorithmic 8 B/elem over its average
                 launch time (HIP events on the launch stream) vs 8 TB/s
   cpu_baseline  the reference's own SeqMdiscr (compiled from source into
-                oracle/_ref) on a bounded sample, on this host
+                oracle/_ref) on the same 2^28 values, on this host, with the
+                other configurations' CPU baselines beside it
 plus the skewed (Zipf 1.5) variant, the per-kernel breakdown, and the C host
 drivers' results: the batched segmented workload (10,000 lognormal segments,
 one graph-replayed launch vs one plain launch per segment) and the serving
@@ -35,6 +36,25 @@ BYTES_PER_ELEM = 8  # one 4-byte read + one 4-byte write per element
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+# The sources the plain partition kernel is compiled from, with the build
+# flags: the HBM traffic in profiles/ was counted for one version of them.
+KERNEL_SOURCES = ["mdiscriminator_amd/csrc/mdiscr_api.hip",
+                  "mdiscriminator_amd/csrc/plain_kernels.hpp",
+                  "mdiscriminator_amd/csrc/device_common.hpp",
+                  "mdiscriminator_amd/csrc/divmagic.h",
+                  "mdiscriminator_amd/csrc/api_internal.hpp",
+                  "mdiscriminator_amd/build.py"]
+
+
+def kernel_source_hash():
+    import hashlib
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
 
 
 def time_steps(ctx, fn, steps, warmup):
@@ -63,53 +83,30 @@ def kernel_times(ctx, fn, reps, names):
     return out
 
 
-def cpu_baseline(n_sample):
-    """The reference's own SeqMdiscr<Mod8> on a bounded sample (5 runs, min and
-    max dropped, mean of the rest, as the reference's sweep does, run.py:49-55)
-    and the OpenMP restatement on all cores."""
-    import numpy as np
-    import oracle
-    from mdiscriminator_amd import workloads as W
-    x = W.uniform_u31(n_sample, SEED)
-    res = {}
-    if oracle.ref_available():
-        ts = []
-        for _ in range(5):
-            t0 = oracle.now()
-            oracle.ref_seq_mdiscr(x, 8, specialized=True)
-            ts.append(oracle.now() - t0)
-        ts = sorted(ts)[1:-1]
-        t = sum(ts) / len(ts)
-        res = {"value": round(n_sample / t / 1e9, 4), "unit": "Gelem/s", "cores": 1,
-               "kind": "reference",
-               "sample": f"SeqMdiscr<Mod8> (cuda_code/SeqMdiscr.h, compiled from source) on "
-                         f"{n_sample} uint32 of the same uniform stream; 5 runs, min/max "
-                         f"dropped, mean {t * 1e3:.1f} ms"}
-    else:
-        ts = []
-        for _ in range(5):
-            t0 = oracle.now()
-            oracle.seq_mdiscr(x, 8, oracle.MASK)
-            ts.append(oracle.now() - t0)
-        ts = sorted(ts)[1:-1]
-        t = sum(ts) / len(ts)
-        res = {"value": round(n_sample / t / 1e9, 4), "unit": "Gelem/s", "cores": 1,
-               "kind": "port", "sample": f"oracle seq_mdiscr on {n_sample} uint32"}
-    # OpenMP restatement of OpenmpMdiscr.h:7-83 (serial tuple scan inside).
-    ncores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    ncores = max(1, min(ncores, 16))
-    n_omp = min(n_sample, 1 << 24)
-    xo = x[:n_omp]
-    ts = []
-    for _ in range(3):
-        t0 = oracle.now()
-        oracle.openmp_mdiscr(xo, 8, oracle.MASK, nthreads=ncores)
-        ts.append(oracle.now() - t0)
-    t = sorted(ts)[1]
-    res["openmp"] = {"value": round(n_omp / t / 1e9, 4), "unit": "Gelem/s", "cores": ncores,
-                     "kind": "port", "sample": f"OpenmpMdiscr restatement on {n_omp} uint32, "
-                                               f"median of 3"}
-    return res
+def cpu_baseline(quick=False):
+    """CPU baselines on this host (oracle/cpu_baseline.py, in a child process so
+    that OMP_PROC_BIND=close is in place before the OpenMP runtime starts):
+    the reference's own SeqMdiscr (compiled from source into oracle/_ref) on
+    configs 1 and 2 at full size, the OpenMP restatement of OpenmpMdiscr.h on
+    config 2, and the SoA and per-segment restatements on configs 3 and 4."""
+    import subprocess
+    env = dict(os.environ)
+    env["OMP_PROC_BIND"] = "close"
+    env.setdefault("OMP_PLACES", "cores")
+    cmd = [sys.executable, "-m", "oracle.cpu_baseline"] + (["--quick"] if quick else [])
+    try:
+        r = subprocess.run(cmd, cwd=ROOT, env=env, capture_output=True, text=True, timeout=900)
+    except subprocess.TimeoutExpired:
+        return {"error": "cpu baseline timed out after 900 s"}
+    sys.stderr.write(r.stderr)
+    if r.returncode != 0:
+        return {"error": f"cpu baseline exit {r.returncode}: {r.stderr[-300:]}"}
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    head = dict(res["config2_seq"])   # the headline workload's own CPU time
+    head["configs"] = {k: v for k, v in res.items() if k.startswith("config")}
+    head["host"] = res["host"]
+    head["method"] = res["method"]
+    return head
 
 
 def run_driver(args, timeout):
@@ -132,12 +129,13 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
-    # The first ~20 steps after the inputs are generated run about 8% slower
-    # than the steady state (0.59 against 0.54-0.55 ms per step at 2^28, m=8):
-    # the default warm-up covers that ramp.
+    # A few steps into this workload the device lowers its shader clock (2.4 to
+    # about 1.7 GHz) and raises it again over some 25 steps; the partition
+    # kernel runs about 15% slower meanwhile.  The default warm-up is past that.
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=1 << 26)
+    ap.add_argument("--cpu-quick", action="store_true",
+                    help="CPU baselines on small samples (a smoke run)")
     ap.add_argument("--no-drivers", action="store_true",
                     help="skip the batched-segment and serving drivers")
     ap.add_argument("--serve-requests", type=int, default=50000)
@@ -227,10 +225,47 @@ def main():
         copy_ms.append(e0.elapsed_time(e1))
     copy_ms = sorted(copy_ms)[len(copy_ms) // 2]
     copy_gbs = N * BYTES_PER_ELEM / (copy_ms * 1e-3) / 1e9
-    traffic = None
+    traffic, traffic_note = None, "no counter profile"
     tpath = os.path.join(ROOT, "profiles", "traffic_partition_2p28_m8.json")
     if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get("bytes_per_launch")
+        tp = json.load(open(tpath))
+        if tp.get("source_hash") == kernel_source_hash():
+            traffic = tp.get("bytes_per_launch")
+            traffic_note = f"{tp.get('kernel')}, counters of the same kernel sources"
+        else:
+            traffic_note = ("stale: the partition kernel's sources changed since the counter "
+                            f"runs (profile {tp.get('source_hash')}, now {kernel_source_hash()})")
+
+    # The plain path across the class axis of the reference's sweep
+    # (cuda_code/run.py:66-82) and a magic-divide Mod<N>: 2^28 uniform keys.
+    sweep = {}
+    if not args.no_drivers:
+        for label, dis in (("m2", md.Mask(2)), ("m4", md.Mask(4)), ("m16", md.Mask(16)),
+                           ("m32", md.Mask(32)), ("m64", md.Mask(64)), ("mod7", md.Mod(7))):
+            def sweep_step():
+                ctx.partition_u32(d_in, d_out, dis, stream=stream)
+            dt_m = time_steps(ctx, sweep_step, 10, 2)
+            ktm = kernel_times(ctx, sweep_step, 10, kernels)
+            ms_m = dt_m / 10 * 1e3
+            sweep[label] = {"ms_per_call": round(ms_m, 4), "gelem_s": round(N / ms_m / 1e6, 2),
+                            "partition_frac": round(N * BYTES_PER_ELEM / (ktm["partition"] * 1e-3)
+                                                    / 1e9 / PEAK_HBM_GBS, 4),
+                            "kernels_ms": {k: round(v, 5) for k, v in ktm.items() if v}}
+
+    # Calls of at most one tile (one launch each), stream-ordered back to back.
+    small = {}
+    if not args.no_drivers:
+        for n_small in (100, 4096):
+            xs = d_in[:n_small]
+            os_ = d_out[:n_small]
+            cs = torch.empty(8, dtype=torch.int32, device="cuda:0")
+
+            def small_step():
+                ctx.partition_u32(xs, os_, md.Mod8, d_counts=cs, stream=stream)
+            reps = 2000
+            dt_sm = time_steps(ctx, small_step, reps, 50)
+            small[f"n{n_small}"] = {"us_per_call": round(dt_sm / reps * 1e6, 2),
+                                    "calls": reps}
 
     # Large-m struct-of-arrays records: 2^26 x (key, float32 payload bits, index).
     soa = {}
@@ -304,8 +339,8 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        log(f"cpu baseline on a sample of {args.cpu_sample}")
-        cpu = cpu_baseline(args.cpu_sample)
+        log("cpu baselines (configs 1-4) on this host")
+        cpu = cpu_baseline(args.cpu_quick)
 
     whole_gbs = N * BYTES_PER_ELEM / (ms_step * 1e-3) / 1e9
     line = {
@@ -325,9 +360,13 @@ def main():
                                "class = x & 7 (Mod8)",
                    "n": N, "m": M, "seed": SEED, "parallelism": "1 GPU"},
         "hbm_frac_whole_path": round(whole_gbs / PEAK_HBM_GBS, 4),
+        # the same step against the minimum traffic of two passes (the count
+        # reads every element before any can be placed): 12 B/elem
+        "hbm_frac_whole_path_12B": round(whole_gbs * 1.5 / PEAK_HBM_GBS, 4),
         "roofline": {"bound": "hbm", "kernel": "partition",
                      "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+                     "traffic_source": traffic_note,
                      "algorithmic_bytes_per_launch": N * BYTES_PER_ELEM,
                      "avg_launch_ms": round(part_ms, 5)},
         "copy_ceiling": {"kernel": "elementwise copy of the same buffers (torch.neg), "
@@ -338,6 +377,8 @@ def main():
         "kernels_ms": {k: round(v, 5) for k, v in kt.items() if v},
         "zipf_variant": {"value": round(N * args.steps / dtz / 1e9, 3), "unit": "Gelem/s",
                          "kernels_ms": {k: round(v, 5) for k, v in ktz.items() if v}},
+        "plain_across_m": sweep or None,
+        "small_calls": small or None,
         "soa_records_2p26": soa or None,
         "general": general or None,
         "batched_segments": seg,

@@ -88,6 +88,23 @@ mdiscr_status mdiscr_partition_u32(mdiscr_ctx* ctx, const uint32_t* d_in,
                                    mdiscr_discr d, uint32_t* d_counts,
                                    void* stream);
 
+/* Where mdiscr_partition_u32_ex writes counts and offsets. */
+#define MDISCR_OUT_HOST 1u /* host memory; the call returns once the device work is done */
+
+/* The plain partition with its class offsets beside the counts.  counts (m
+ * entries) and offsets (m+1 entries, offsets[k] = counts[0] + ... +
+ * counts[k-1], offsets[m] = n: the exclusive scan of SeqMdiscr.h:34-38) may
+ * each be NULL.  Without flags they are device memory, written
+ * stream-ordered like d_counts above.  With MDISCR_OUT_HOST they are host
+ * memory and the call waits for `stream`, as the reference returns its counts
+ * by value after a device-to-host copy (MdiscrHost_optimized_clean.cu.h:87-92,
+ * 149). */
+mdiscr_status mdiscr_partition_u32_ex(mdiscr_ctx* ctx, const uint32_t* d_in,
+                                      uint32_t* d_out, uint64_t n,
+                                      mdiscr_discr d, uint32_t* counts,
+                                      uint64_t* offsets, uint32_t flags,
+                                      void* stream);
+
 /* Reference-shaped synchronous call: like mdiscr<DISCR>(n, num_hwd_thds,
  * d_in, d_out), the class counts are returned in host memory h_counts (m
  * entries) after the device work has finished.  num_hwd_thds is accepted
@@ -262,6 +279,13 @@ mdiscr_status mdiscr_server_wait(mdiscr_server* srv, uint64_t ticket, uint32_t**
                                  uint32_t* h_counts, double* latency_s);
 /* Returns the request's output buffer to the budget. */
 mdiscr_status mdiscr_server_release(mdiscr_server* srv, uint64_t ticket);
+/* Withdraws a request that is not yet on the GPU (queued for budget or
+ * admitted and waiting for a slot): its output block returns to the budget,
+ * the ticket becomes unknown, and a thread blocked in mdiscr_server_wait on it
+ * returns MDISCR_EINVAL.  MDISCR_EBUSY once the request has been launched or
+ * has finished (wait for it, then release it).  No reference counterpart: the
+ * reference has no request queue. */
+mdiscr_status mdiscr_server_cancel(mdiscr_server* srv, uint64_t ticket);
 mdiscr_status mdiscr_server_stats_get(mdiscr_server* srv, mdiscr_server_stats* out);
 
 /* ---- output format (HelpersCommon.h:295-311) --------------------------- */

@@ -189,6 +189,27 @@ class Context:
                                            _stream(stream))
         self._check(st, "mdiscr_partition_u32")
 
+    def partition_u32_ex(self, d_in, d_out, discr, counts=None, offsets=None, host=False,
+                         n=None, stream=None):
+        """The plain partition with counts (m) and offsets (m+1, uint64).
+        host=False: counts/offsets are device tensors, written stream-ordered.
+        host=True: they are returned as Python lists after the device work."""
+        n = d_in.numel() if n is None else n
+        if host:
+            hc = (ctypes.c_uint32 * discr.m)()
+            ho = (ctypes.c_uint64 * (discr.m + 1))()
+            st = self.lib.mdiscr_partition_u32_ex(self.h, _ptr(d_in, 4 * n), _ptr(d_out, 4 * n),
+                                                  n, discr.c(), hc, ho, _lib.OUT_HOST,
+                                                  _stream(stream))
+            self._check(st, "mdiscr_partition_u32_ex")
+            return list(hc), list(ho)
+        st = self.lib.mdiscr_partition_u32_ex(self.h, _ptr(d_in, 4 * n), _ptr(d_out, 4 * n), n,
+                                              discr.c(), _ptr(counts, 4 * discr.m),
+                                              _ptr(offsets, 8 * (discr.m + 1)), 0,
+                                              _stream(stream))
+        self._check(st, "mdiscr_partition_u32_ex")
+        return None
+
     def mdiscr(self, discr, num_elems, num_hwd_thds, d_in, d_out):
         """Reference-shaped synchronous call; returns the m class counts."""
         counts = (ctypes.c_uint32 * discr.m)()
@@ -379,6 +400,16 @@ class Server:
         if st != _lib.OK:
             raise MdiscrError(st, "mdiscr_server_release")
         self._m.pop(ticket, None)
+
+    def cancel(self, ticket):
+        """Withdraws a request not yet on the GPU; returns the status code
+        (OK, or EBUSY once it has been launched)."""
+        st = self.lib.mdiscr_server_cancel(self.h, ticket)
+        if st == _lib.OK:
+            self._m.pop(ticket, None)
+        elif st != _lib.EBUSY:
+            raise MdiscrError(st, "mdiscr_server_cancel")
+        return st
 
     def stats(self):
         s = _lib.ServerStats()

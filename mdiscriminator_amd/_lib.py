@@ -14,6 +14,7 @@ DISCR_MOD, DISCR_MASK = 0, 1
 PRED_LT, PRED_LE, PRED_GT, PRED_GE, PRED_EQ, PRED_NE, PRED_BITS, PRED_RANGE = range(8)
 MAX_CLASSES = 64
 MAX_ELEMS = 0xFFFF0000
+OUT_HOST = 1
 
 
 class MdiscrDiscr(ctypes.Structure):
@@ -64,6 +65,7 @@ SIGNATURES = {
     "mdiscr_last_error": (ctypes.c_char_p, [_vp]),
     "mdiscr_partition_u32": (_int, [_vp, _vp, _vp, _u64, MdiscrDiscr, _vp, _vp]),
     "mdiscr_u32": (_int, [_vp, _u64, _u32, _vp, _vp, MdiscrDiscr, _vp]),
+    "mdiscr_partition_u32_ex": (_int, [_vp, _vp, _vp, _u64, MdiscrDiscr, _vp, _vp, _u32, _vp]),
     "mdiscr_partition_soa": (_int, [_vp, _u32, _vp, _vp, _u32, _vp, _u64, MdiscrDiscr, _vp, _vp]),
     "mdiscr_partition_arrays": (_int, [_vp, _vp, _vp, _u64, _u32, MdiscrDiscr, _vp, _vp]),
     "mdiscr_filter_u32": (_int, [_vp, _vp, _vp, _u64, MdiscrPred, _vp, _vp]),
@@ -82,6 +84,7 @@ SIGNATURES = {
     "mdiscr_server_wait": (_int, [_vp, _u64, ctypes.POINTER(_vp), _vp,
                                   ctypes.POINTER(ctypes.c_double)]),
     "mdiscr_server_release": (_int, [_vp, _u64]),
+    "mdiscr_server_cancel": (_int, [_vp, _u64]),
     "mdiscr_server_stats_get": (_int, [_vp, _vp]),
     "mdiscr_timing_enable": (_int, [_vp, _int]),
     "mdiscr_timing_read": (_int, [_vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),

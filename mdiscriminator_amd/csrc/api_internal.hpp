@@ -22,6 +22,11 @@ struct mdiscr_ctx {
     int num_cu = 256;
     void* ws = nullptr;        // grow-only workspace
     size_t ws_bytes = 0;
+    // Buffers replaced by a larger one.  Work queued before the growth (or a
+    // graph captured with them) may still read them, so they are freed only
+    // when the context is destroyed; growth at least doubles, which bounds
+    // them by the live buffer's size.
+    std::vector<void*> retired;
     void* small = nullptr;     // 64 KiB of device scratch for count readbacks
     bool timing = false;
     std::vector<TimedLaunch> pending;
@@ -33,8 +38,20 @@ struct mdiscr_ctx {
 mdiscr_status ctx_fail(mdiscr_ctx* ctx, mdiscr_status s, const char* fmt, ...);
 mdiscr_status ctx_hip(mdiscr_ctx* ctx, hipError_t e, const char* what);
 mdiscr_status ctx_workspace(mdiscr_ctx* ctx, size_t bytes, void** out);
+// Grow-only device buffer: a larger request allocates anew and retires the
+// old buffer to ctx->retired (no device-wide wait, no free while in use).
+mdiscr_status ctx_grow(mdiscr_ctx* ctx, void** buf, size_t* have, size_t need);
+// Device and stream checks every launching entry point makes first: the
+// context's device is made current, and a stream that is being captured into
+// a graph is refused with MDISCR_EUNSUPPORTED before anything is allocated,
+// synchronised or launched.
+mdiscr_status ctx_begin(mdiscr_ctx* ctx, hipStream_t s);
 mdiscr_status ctx_small_buffer(mdiscr_ctx* ctx, size_t bytes, void** out);
 void ctx_release_extra(mdiscr_ctx* ctx);
+// Sizes the batch path's workspace and pinned descriptor staging for batches
+// of up to max_reqs requests and max_elems elements (any m), so that such
+// batches never grow a buffer.
+mdiscr_status ctx_reserve_batch(mdiscr_ctx* ctx, uint64_t max_elems, uint32_t max_reqs);
 void timing_begin(mdiscr_ctx* ctx, hipStream_t s, TimedLaunch& t);
 void timing_end(mdiscr_ctx* ctx, hipStream_t s, TimedLaunch& t, const char* name);
 mdiscr_status check_discr(mdiscr_ctx* ctx, mdiscr_discr d);

@@ -364,7 +364,7 @@ extern "C" mdiscr_status mdiscr_partition_arrays(mdiscr_ctx* ctx, const uint32_t
     if (n && ranges_overlap(d_in, d_out, n * len * 4))
         return ctx_fail(ctx, MDISCR_EINVAL, "output overlaps input");
     hipStream_t s = (hipStream_t)stream;
-    st = ctx_hip(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    st = ctx_begin(ctx, s);
     if (st) return st;
     if (n == 0) {
         if (d_counts) {

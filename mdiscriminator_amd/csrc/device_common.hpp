@@ -83,6 +83,21 @@ __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t lane) {
     return x - v;
 }
 
+// Inclusive prefix sum across the 64 lanes of a wave, on DPP moves only (no
+// LDS traffic): four row shifts scan each row of 16 lanes, then the last lane
+// of row 0 is added into row 1 and of row 2 into row 3 (row_bcast:15), and
+// the last lane of row 1 into rows 2 and 3 (row_bcast:31).  Lanes a DPP move
+// cannot source from (or whose row is masked off) add 0.
+__device__ __forceinline__ uint32_t wave_incl_scan_dpp(uint32_t x) {
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false);  // row_shr:1
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false);  // row_shr:2
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false);  // row_shr:4
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false);  // row_shr:8
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    return x;
+}
+
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 #pragma unroll
     for (int d = kWave / 2; d >= 1; d >>= 1) v += __shfl_xor(v, d, kWave);

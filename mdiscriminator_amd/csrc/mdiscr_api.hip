@@ -6,6 +6,7 @@
 // MdiscrHost_optimized_clean.cu.h:43-45,145-147).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -38,26 +39,52 @@ mdiscr_status ctx_hip(mdiscr_ctx* ctx, hipError_t e, const char* what) {
     return ctx_fail(ctx, MDISCR_EDEVICE, "%s: %s", what, hipGetErrorString(e));
 }
 
-mdiscr_status ctx_workspace(mdiscr_ctx* ctx, size_t bytes, void** out) {
-    bytes = (bytes + 255) & ~(size_t)255;
-    if (bytes > ctx->ws_bytes) {
-        if (ctx->ws) {
-            // The old workspace may still be in use by queued work.
-            hipError_t e = hipDeviceSynchronize();
-            if (e != hipSuccess) return ctx_hip(ctx, e, "hipDeviceSynchronize");
-            hipFree(ctx->ws);
-            ctx->ws = nullptr;
-            ctx->ws_bytes = 0;
-        }
-        hipError_t e = hipMalloc(&ctx->ws, bytes);
-        if (e != hipSuccess) {
-            (void)hipGetLastError();
-            return ctx_fail(ctx, MDISCR_ENOMEM, "workspace of %zu bytes: %s", bytes,
-                            hipGetErrorString(e));
-        }
-        ctx->ws_bytes = bytes;
+mdiscr_status ctx_grow(mdiscr_ctx* ctx, void** buf, size_t* have, size_t need) {
+    need = (need + 255) & ~(size_t)255;
+    if (need <= *have) return MDISCR_OK;
+    // At least double, so that the retired buffers stay below the live one.
+    const size_t bytes = std::max(need, *have * 2);
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return ctx_fail(ctx, MDISCR_ENOMEM, "%zu-byte device buffer: %s", bytes,
+                        hipGetErrorString(e));
     }
+    // Queued work may still use the old buffer: it is freed with the context,
+    // never here (hipFree would also wait for the whole device, stalling every
+    // other stream).
+    if (*buf) ctx->retired.push_back(*buf);
+    *buf = p;
+    *have = bytes;
+    return MDISCR_OK;
+}
+
+mdiscr_status ctx_workspace(mdiscr_ctx* ctx, size_t bytes, void** out) {
+    mdiscr_status st = ctx_grow(ctx, &ctx->ws, &ctx->ws_bytes, bytes);
+    if (st) return st;
     *out = ctx->ws;
+    return MDISCR_OK;
+}
+
+mdiscr_status ctx_begin(mdiscr_ctx* ctx, hipStream_t s) {
+    mdiscr_status st = ctx_hip(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    if (st) return st;
+    // A call under stream capture would record its kernels into the caller's
+    // graph with the context workspace's address baked in, and could grow
+    // (reallocate) that workspace inside the capture.  Neither is supported:
+    // the call is refused before it touches the device.
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    const hipError_t e = hipStreamIsCapturing(s, &cs);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return ctx_fail(ctx, MDISCR_EUNSUPPORTED, "stream capture state unknown: %s",
+                        hipGetErrorString(e));
+    }
+    if (cs != hipStreamCaptureStatusNone)
+        return ctx_fail(ctx, MDISCR_EUNSUPPORTED,
+                        "the stream is being captured into a graph; calls under capture are "
+                        "not supported");
     return MDISCR_OK;
 }
 
@@ -121,6 +148,7 @@ mdiscr_status mdiscr_ctx_destroy(mdiscr_ctx* ctx) {
         hipEventDestroy(t.b);
     }
     if (ctx->ws) hipFree(ctx->ws);
+    for (void* p : ctx->retired) hipFree(p);
     ctx_release_extra(ctx);
     delete ctx;
     return MDISCR_OK;
@@ -226,91 +254,223 @@ constexpr bool kPrefetch = MDISCR_PLAIN_PREFETCH != 0;
 using PlainGeom = Geom<kNW, kR, kCT>;
 constexpr uint32_t kTile = PlainGeom::kTile;
 
-template <int NBITS, class Discr>
-mdiscr_status launch_plain(mdiscr_ctx* ctx, const uint32_t* in, uint32_t* out, uint32_t n,
-                           Discr dis, uint32_t m, uint32_t* user_counts, hipStream_t s,
-                           uint32_t* ws) {
-    const uint32_t T = (uint32_t)(((uint64_t)n + kTile - 1) / kTile);
-    const uint32_t ld = (T + 3u) & ~3u;     // 16-byte aligned class rows
-    const uint32_t Kc = (T + kCCT - 1) / kCCT;
-    const uint32_t Kp = (T + kCT - 1) / kCT;
-    uint32_t* counts = ws;                      // m * ld
-    uint32_t* prefix = ws + (size_t)m * ld;     // m * ld
-    uint32_t* totals = ws + (size_t)2 * m * ld; // m
-    const bool vec = ((uintptr_t)in & 15u) == 0;
-    TimedLaunch tl;
-    if (T == 1) {
-        // One tile: the partition counts it itself (3 launches -> 1; a call on
-        // 4096 elements or fewer is bound by launch latency, not bandwidth).
-        timing_begin(ctx, s, tl);
-        hipLaunchKernelGGL((k_partition<kNW, kR, kCT, NBITS, kPrefetch, MDISCR_PLAIN_WB,
-                                        MDISCR_PLAIN_MINW, Discr, true>),
-                           dim3(1), dim3(kNW * kWave), 0, s, in, out, n, dis, nullptr, nullptr,
-                           0u, m, user_counts);
-        timing_end(ctx, s, tl, "partition");
-        return ctx_hip(ctx, hipGetLastError(), "plain partition launch");
+// The quad path (16-byte loads, packed-counter ranks) serves up to 16
+// classes on a 16-byte aligned input, in tiles of its own size; the
+// ballot-ranked kernel serves the rest.  MDISCR_PLAIN_QUAD = 2 also stages
+// class runs aligned to the output for 16-byte stores, 1 keeps 4-byte
+// stores, 0 keeps the ballot-ranked kernel everywhere.
+#ifndef MDISCR_PLAIN_QUAD
+#define MDISCR_PLAIN_QUAD 1
+#endif
+#ifndef MDISCR_QUAD_NW
+#define MDISCR_QUAD_NW 4
+#endif
+#ifndef MDISCR_QUAD_R
+#define MDISCR_QUAD_R 32
+#endif
+// Waves of the quad path's count workgroup: one tile each, 16 rounds of
+// 16-byte loads per lane (more rounds per lane read slower).
+#ifndef MDISCR_QUAD_COUNT_NW
+#define MDISCR_QUAD_COUNT_NW 8
+#endif
+// Quad partition workgroups resident per CU (0: as many as fit).  The kernel
+// needs few registers, but more resident tiles interleave more read and
+// write streams at once, which HBM serves more slowly: the launch pads its
+// LDS allocation to hold the count at this many.  2^28 keys, m = 8, 8192-key
+// tiles, interleaved A/B: 0.360 ms at 2 per CU against 0.377 ms for 4096-key
+// tiles at 4 per CU and 0.439 / 0.377 / 0.384 / 0.402 ms at 2 / 4 / 5 / 6.
+#ifndef MDISCR_PLAIN_WG_PER_CU
+#define MDISCR_PLAIN_WG_PER_CU 2
+#endif
+constexpr bool kQuadOut16 = MDISCR_PLAIN_QUAD == 2;
+constexpr int kQNW = MDISCR_QUAD_NW;
+constexpr int kQR = MDISCR_QUAD_R;
+constexpr uint32_t kQTile = (uint32_t)kQNW * kWave * kQR;
+constexpr int kQCNW = MDISCR_QUAD_COUNT_NW;
+constexpr int kQCR = (int)(kQTile / (kQCNW * kWave));
+static_assert(kQCR * kQCNW * kWave == (int)kQTile, "count and partition tiles agree");
+static_assert(kQTile >= kTile, "the workspace is sized by the smaller tile");
+
+// Dynamic LDS bytes that make `kernel` (static LDS as compiled) fit at most
+// `per_cu` workgroups per CU: the CU's LDS divided by per_cu, less the static part.
+template <typename K>
+size_t lds_pad_for(K kernel, int per_cu) {
+    if (per_cu <= 0) return 0;
+    hipFuncAttributes fa{};
+    int dev = 0, lds_cu = 0;
+    if (hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(kernel)) != hipSuccess ||
+        hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) !=
+            hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
     }
+    const size_t target = (size_t)lds_cu / (size_t)per_cu;
+    const size_t pad = target > fa.sharedSizeBytes ? target - fa.sharedSizeBytes : 0;
+    return pad & ~(size_t)255;
+}
+
+// The count and tile scan of a call in tiles of CNW * 64 * CR elements;
+// counts, prefix and totals are laid out in the workspace as above.
+template <int CNW, int CR, int NBITS, class Discr>
+void launch_count_scan(mdiscr_ctx* ctx, const uint32_t* in, uint32_t n, Discr dis, uint32_t m,
+                       uint32_t* user_counts, hipStream_t s, uint32_t* counts, uint32_t* prefix,
+                       uint32_t* totals, uint32_t T, uint32_t ld) {
+    TimedLaunch tl;
+    const uint32_t Kc = (T + kCCT - 1) / kCCT;
     timing_begin(ctx, s, tl);
-    if (vec)
-        hipLaunchKernelGGL((k_chunk_count<kNW, kR, kCCT, NBITS, true, Discr>), dim3(Kc),
-                           dim3(kNW * kWave), 0, s, in, n, dis, counts, ld, m);
+    if (((uintptr_t)in & 15u) == 0)
+        hipLaunchKernelGGL((k_chunk_count<CNW, CR, kCCT, NBITS, true, Discr>), dim3(Kc),
+                           dim3(CNW * kWave), 0, s, in, n, dis, counts, ld, m);
     else
-        hipLaunchKernelGGL((k_chunk_count<kNW, kR, kCCT, NBITS, false, Discr>), dim3(Kc),
-                           dim3(kNW * kWave), 0, s, in, n, dis, counts, ld, m);
+        hipLaunchKernelGGL((k_chunk_count<CNW, CR, kCCT, NBITS, false, Discr>), dim3(Kc),
+                           dim3(CNW * kWave), 0, s, in, n, dis, counts, ld, m);
     timing_end(ctx, s, tl, "chunk_count");
     timing_begin(ctx, s, tl);
     launch_tile_scan(counts, prefix, T, ld, m, totals, user_counts, nullptr, s);
     timing_end(ctx, s, tl, "tile_scan");
+}
+
+// The quad path: count, scan and partition in kQTile tiles, or the
+// partition alone for an input of one tile (it counts that tile itself).
+template <int NBITS, class Discr>
+mdiscr_status launch_plain_quad(mdiscr_ctx* ctx, const uint32_t* in, uint32_t* out, uint32_t n,
+                                Discr dis, uint32_t m, uint32_t* user_counts, uint64_t* offsets,
+                                hipStream_t s, uint32_t* ws) {
+    const uint32_t T = (uint32_t)(((uint64_t)n + kQTile - 1) / kQTile);
+    const uint32_t ld = (T + 3u) & ~3u;
+    uint32_t* counts = ws;
+    uint32_t* prefix = ws + (size_t)m * ld;
+    uint32_t* totals = ws + (size_t)2 * m * ld;
+    TimedLaunch tl;
+    if (T == 1) {
+        timing_begin(ctx, s, tl);
+        hipLaunchKernelGGL((k_partition_quad<kQNW, kQR, NBITS, MDISCR_PLAIN_WB, kQuadOut16,
+                                             MDISCR_PLAIN_MINW, Discr, true>),
+                           dim3(1), dim3(kQNW * kWave), 0, s, in, out, n, dis, nullptr, nullptr,
+                           0u, m, user_counts, offsets);
+        timing_end(ctx, s, tl, "partition");
+        return ctx_hip(ctx, hipGetLastError(), "plain partition launch");
+    }
+    launch_count_scan<kQCNW, kQCR, NBITS>(ctx, in, n, dis, m, user_counts, s, counts, prefix,
+                                          totals, T, ld);
+    constexpr auto kern = k_partition_quad<kQNW, kQR, NBITS, MDISCR_PLAIN_WB, kQuadOut16,
+                                           MDISCR_PLAIN_MINW, Discr>;
+    static const size_t pad = lds_pad_for(kern, MDISCR_PLAIN_WG_PER_CU);
+    timing_begin(ctx, s, tl);
+    hipLaunchKernelGGL(kern, dim3(T), dim3(kQNW * kWave), pad, s, in, out, n, dis, prefix, totals,
+                       ld, m, nullptr, offsets);
+    timing_end(ctx, s, tl, "partition");
+    return ctx_hip(ctx, hipGetLastError(), "plain partition launch");
+}
+
+template <int NBITS, class Discr>
+mdiscr_status launch_plain(mdiscr_ctx* ctx, const uint32_t* in, uint32_t* out, uint32_t n,
+                           Discr dis, uint32_t m, uint32_t* user_counts, uint64_t* offsets,
+                           hipStream_t s, uint32_t* ws) {
+    if constexpr (NBITS <= 4 && MDISCR_PLAIN_QUAD != 0) {
+        if (((uintptr_t)in & 15u) == 0)
+            return launch_plain_quad<NBITS>(ctx, in, out, n, dis, m, user_counts, offsets, s, ws);
+    }
+    const uint32_t T = (uint32_t)(((uint64_t)n + kTile - 1) / kTile);
+    const uint32_t ld = (T + 3u) & ~3u;     // 16-byte aligned class rows
+    const uint32_t Kp = (T + kCT - 1) / kCT;
+    uint32_t* counts = ws;                      // m * ld
+    uint32_t* prefix = ws + (size_t)m * ld;     // m * ld
+    uint32_t* totals = ws + (size_t)2 * m * ld; // m
+    TimedLaunch tl;
+    if (T == 1) {
+        // One tile: the partition counts it itself (3 launches -> 1; a call on
+        // one tile or less is bound by launch latency, not bandwidth).
+        timing_begin(ctx, s, tl);
+        hipLaunchKernelGGL((k_partition<kNW, kR, kCT, NBITS, kPrefetch, MDISCR_PLAIN_WB,
+                                        MDISCR_PLAIN_MINW, Discr, true>),
+                           dim3(1), dim3(kNW * kWave), 0, s, in, out, n, dis, nullptr, nullptr,
+                           0u, m, user_counts, offsets);
+        timing_end(ctx, s, tl, "partition");
+        return ctx_hip(ctx, hipGetLastError(), "plain partition launch");
+    }
+    launch_count_scan<kNW, kR, NBITS>(ctx, in, n, dis, m, user_counts, s, counts, prefix, totals,
+                                      T, ld);
     timing_begin(ctx, s, tl);
     hipLaunchKernelGGL((k_partition<kNW, kR, kCT, NBITS, kPrefetch, MDISCR_PLAIN_WB, MDISCR_PLAIN_MINW, Discr>),
                        dim3(Kp), dim3(kNW * kWave), 0, s, in, out, n, dis, prefix, totals, ld, m,
-                       nullptr);
+                       nullptr, offsets);
     timing_end(ctx, s, tl, "partition");
     return ctx_hip(ctx, hipGetLastError(), "plain partition launch");
 }
 
 template <class Discr>
 mdiscr_status dispatch_plain(mdiscr_ctx* ctx, const uint32_t* in, uint32_t* out, uint32_t n,
-                             Discr dis, uint32_t m, uint32_t* user_counts, hipStream_t s,
-                             uint32_t* ws) {
-    const uint32_t nb = class_bits(m);
-    switch (nb) {
-        case 0: return launch_plain<0>(ctx, in, out, n, dis, m, user_counts, s, ws);
-        case 1: return launch_plain<1>(ctx, in, out, n, dis, m, user_counts, s, ws);
-        case 2: return launch_plain<2>(ctx, in, out, n, dis, m, user_counts, s, ws);
-        case 3: return launch_plain<3>(ctx, in, out, n, dis, m, user_counts, s, ws);
-        case 4: return launch_plain<4>(ctx, in, out, n, dis, m, user_counts, s, ws);
-        case 5: return launch_plain<5>(ctx, in, out, n, dis, m, user_counts, s, ws);
-        case 6: return launch_plain<6>(ctx, in, out, n, dis, m, user_counts, s, ws);
+                             Discr dis, uint32_t m, uint32_t* user_counts, uint64_t* offsets,
+                             hipStream_t s, uint32_t* ws) {
+    switch (class_bits(m)) {
+        case 0: return launch_plain<0>(ctx, in, out, n, dis, m, user_counts, offsets, s, ws);
+        case 1: return launch_plain<1>(ctx, in, out, n, dis, m, user_counts, offsets, s, ws);
+        case 2: return launch_plain<2>(ctx, in, out, n, dis, m, user_counts, offsets, s, ws);
+        case 3: return launch_plain<3>(ctx, in, out, n, dis, m, user_counts, offsets, s, ws);
+        case 4: return launch_plain<4>(ctx, in, out, n, dis, m, user_counts, offsets, s, ws);
+        case 5: return launch_plain<5>(ctx, in, out, n, dis, m, user_counts, offsets, s, ws);
+        case 6: return launch_plain<6>(ctx, in, out, n, dis, m, user_counts, offsets, s, ws);
     }
     return ctx_fail(ctx, MDISCR_EUNSUPPORTED, "m=%u", m);
 }
 
-// Argument checks, the n = 0 case and the workspace of one plain-path call.
-// On success *ws is the workspace, or null when nothing is left to launch.
-mdiscr_status plain_prologue(mdiscr_ctx* ctx, const uint32_t* d_in, uint32_t* d_out, uint64_t n,
-                             uint32_t m, uint32_t* d_counts, hipStream_t s, uint32_t** ws) {
-    *ws = nullptr;
+// Argument, device and stream checks of one plain-path call, and its n = 0
+// case (zero counts and offsets).  *done is set when nothing is left to launch.
+mdiscr_status plain_check(mdiscr_ctx* ctx, const uint32_t* d_in, uint32_t* d_out, uint64_t n,
+                          uint32_t m, uint32_t* d_counts, uint64_t* d_offsets, hipStream_t s,
+                          bool* done) {
+    *done = true;
     if (n > MDISCR_MAX_ELEMS) return ctx_fail(ctx, MDISCR_EINVAL, "n=%llu too large", (unsigned long long)n);
     if (n && (!d_in || !d_out)) return ctx_fail(ctx, MDISCR_EINVAL, "null data pointer");
     if (n && ranges_overlap(d_in, d_out, n * 4))
         return ctx_fail(ctx, MDISCR_EINVAL, "output overlaps input");
-    mdiscr_status st = ctx_hip(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    mdiscr_status st = ctx_begin(ctx, s);
     if (st) return st;
     if (n == 0) {
-        if (d_counts) {
+        if (d_counts)
             hipLaunchKernelGGL(k_fill_u32<>, dim3(1), dim3(64), 0, s, d_counts, 0u, m);
-            return ctx_hip(ctx, hipGetLastError(), "fill launch");
-        }
-        return MDISCR_OK;
+        if (d_offsets)  // m+1 zero 64-bit words
+            hipLaunchKernelGGL(k_fill_u32<>, dim3(1), dim3(64), 0, s,
+                               reinterpret_cast<uint32_t*>(d_offsets), 0u, 2 * (m + 1));
+        return ctx_hip(ctx, hipGetLastError(), "fill launch");
     }
+    *done = false;
+    return MDISCR_OK;
+}
+
+// Workspace bytes of a plain-path call: per-tile class counts and their
+// prefix (class-major rows of ld entries), then the m class totals.
+size_t plain_ws_bytes(uint64_t n, uint32_t m) {
     const uint64_t T = (n + kTile - 1) / kTile;
     const uint64_t ld = (T + 3) & ~3ull;
-    void* w = nullptr;
-    st = ctx_workspace(ctx, (2 * ld + 1) * m * sizeof(uint32_t), &w);  // counts, prefix, totals
+    return (2 * ld + 1) * m * sizeof(uint32_t);
+}
+
+// The launches of one checked plain-path call with its workspace.
+mdiscr_status plain_run(mdiscr_ctx* ctx, const uint32_t* d_in, uint32_t* d_out, uint64_t n,
+                        mdiscr_discr d, uint32_t* d_counts, uint64_t* d_offsets, hipStream_t s,
+                        uint32_t* ws) {
+    if (d.kind == MDISCR_DISCR_MASK || (d.m & (d.m - 1)) == 0)
+        return dispatch_plain(ctx, d_in, d_out, (uint32_t)n, DiscrMask{d.m - 1}, d.m, d_counts,
+                              d_offsets, s, ws);
+    return dispatch_plain(ctx, d_in, d_out, (uint32_t)n, DiscrMod{mdiscr_divmagic_make(d.m)}, d.m,
+                          d_counts, d_offsets, s, ws);
+}
+
+// Checks, workspace and launches of a stream-ordered plain call.
+mdiscr_status plain_call(mdiscr_ctx* ctx, const uint32_t* d_in, uint32_t* d_out, uint64_t n,
+                         mdiscr_discr d, uint32_t* d_counts, uint64_t* d_offsets, hipStream_t s) {
+    mdiscr_status st = check_discr(ctx, d);
     if (st) return st;
-    *ws = static_cast<uint32_t*>(w);
-    return MDISCR_OK;
+    bool done = true;
+    st = plain_check(ctx, d_in, d_out, n, d.m, d_counts, d_offsets, s, &done);
+    if (st || done) return st;
+    void* w = nullptr;
+    st = ctx_workspace(ctx, plain_ws_bytes(n, d.m), &w);
+    if (st) return st;
+    return plain_run(ctx, d_in, d_out, n, d, d_counts, d_offsets, s, static_cast<uint32_t*>(w));
 }
 
 }  // namespace
@@ -378,17 +538,38 @@ mdiscr_status mdiscr_partition_u32(mdiscr_ctx* ctx, const uint32_t* d_in, uint32
                                    uint64_t n, mdiscr_discr d, uint32_t* d_counts,
                                    void* stream) {
     if (!ctx) return MDISCR_EINVAL;
+    return plain_call(ctx, d_in, d_out, n, d, d_counts, nullptr, (hipStream_t)stream);
+}
+
+mdiscr_status mdiscr_partition_u32_ex(mdiscr_ctx* ctx, const uint32_t* d_in, uint32_t* d_out,
+                                      uint64_t n, mdiscr_discr d, uint32_t* counts,
+                                      uint64_t* offsets, uint32_t flags, void* stream) {
+    if (!ctx) return MDISCR_EINVAL;
+    if (flags & ~(uint32_t)MDISCR_OUT_HOST)
+        return ctx_fail(ctx, MDISCR_EINVAL, "unknown flags 0x%x", flags);
+    hipStream_t s = (hipStream_t)stream;
+    if (!(flags & MDISCR_OUT_HOST)) return plain_call(ctx, d_in, d_out, n, d, counts, offsets, s);
+    // Host destinations: the kernels write device staging in the context's
+    // small buffer (counts at byte 0, offsets at byte 512), copied back after.
     mdiscr_status st = check_discr(ctx, d);
     if (st) return st;
-    hipStream_t s = (hipStream_t)stream;
-    uint32_t* ws = nullptr;
-    st = plain_prologue(ctx, d_in, d_out, n, d.m, d_counts, s, &ws);
-    if (st || !ws) return st;
-    if (d.kind == MDISCR_DISCR_MASK || (d.m & (d.m - 1)) == 0)
-        return dispatch_plain(ctx, d_in, d_out, (uint32_t)n, DiscrMask{d.m - 1}, d.m, d_counts, s,
-                              ws);
-    return dispatch_plain(ctx, d_in, d_out, (uint32_t)n, DiscrMod{mdiscr_divmagic_make(d.m)}, d.m,
-                          d_counts, s, ws);
+    st = ctx_begin(ctx, s);
+    if (st) return st;
+    char* stage = nullptr;
+    st = ctx_small_buffer(ctx, 512 + (MDISCR_MAX_CLASSES + 1) * sizeof(uint64_t), (void**)&stage);
+    if (st) return st;
+    uint32_t* dc = counts ? reinterpret_cast<uint32_t*>(stage) : nullptr;
+    uint64_t* dof = offsets ? reinterpret_cast<uint64_t*>(stage + 512) : nullptr;
+    st = plain_call(ctx, d_in, d_out, n, d, dc, dof, s);
+    if (st) return st;
+    if (counts)
+        st = ctx_hip(ctx, hipMemcpyAsync(counts, dc, d.m * sizeof(uint32_t),
+                                         hipMemcpyDeviceToHost, s), "counts readback");
+    if (!st && offsets)
+        st = ctx_hip(ctx, hipMemcpyAsync(offsets, dof, (d.m + 1) * sizeof(uint64_t),
+                                         hipMemcpyDeviceToHost, s), "offsets readback");
+    if (st) return st;
+    return ctx_hip(ctx, hipStreamSynchronize(s), "hipStreamSynchronize");
 }
 
 mdiscr_status mdiscr_filter_u32(mdiscr_ctx* ctx, const uint32_t* d_in, uint32_t* d_out,
@@ -397,11 +578,14 @@ mdiscr_status mdiscr_filter_u32(mdiscr_ctx* ctx, const uint32_t* d_in, uint32_t*
     uint32_t w[4];
     if (!pred_window(pred, w)) return ctx_fail(ctx, MDISCR_EINVAL, "unknown predicate %u", pred.op);
     hipStream_t s = (hipStream_t)stream;
-    uint32_t* ws = nullptr;
-    mdiscr_status st = plain_prologue(ctx, d_in, d_out, n, 2, d_counts, s, &ws);
-    if (st || !ws) return st;
+    bool done = true;
+    mdiscr_status st = plain_check(ctx, d_in, d_out, n, 2, d_counts, nullptr, s, &done);
+    if (st || done) return st;
+    void* ws = nullptr;
+    st = ctx_workspace(ctx, plain_ws_bytes(n, 2), &ws);
+    if (st) return st;
     return launch_plain<1>(ctx, d_in, d_out, (uint32_t)n, DiscrPred{w[0], w[1], w[2], w[3]}, 2u,
-                           d_counts, s, ws);
+                           d_counts, nullptr, s, static_cast<uint32_t*>(ws));
 }
 
 mdiscr_status mdiscr_u32(mdiscr_ctx* ctx, uint64_t n, uint32_t num_hwd_thds,
@@ -409,16 +593,8 @@ mdiscr_status mdiscr_u32(mdiscr_ctx* ctx, uint64_t n, uint32_t num_hwd_thds,
                          uint32_t* h_counts) {
     (void)num_hwd_thds;
     if (!ctx || !h_counts) return ctx_fail(ctx, MDISCR_EINVAL, "null argument");
-    mdiscr_status st = check_discr(ctx, d);
-    if (st) return st;
-    uint32_t* dc = nullptr;
-    st = ctx_small_buffer(ctx, d.m * sizeof(uint32_t), (void**)&dc);
-    if (st) return st;
-    st = mdiscr_partition_u32(ctx, d_in, d_out, n, d, dc, nullptr);
-    if (st) return st;
-    st = ctx_hip(ctx, hipMemcpy(h_counts, dc, d.m * sizeof(uint32_t), hipMemcpyDeviceToHost),
-                 "counts readback");
-    return st;
+    return mdiscr_partition_u32_ex(ctx, d_in, d_out, n, d, h_counts, nullptr, MDISCR_OUT_HOST,
+                                   nullptr);
 }
 
 mdiscr_status mdiscr_sizes_from_counts(mdiscr_ctx* ctx, const uint32_t* d_counts, uint32_t m,
@@ -426,8 +602,9 @@ mdiscr_status mdiscr_sizes_from_counts(mdiscr_ctx* ctx, const uint32_t* d_counts
     if (!ctx) return MDISCR_EINVAL;
     if (!d_counts || m == 0 || m > MDISCR_MAX_CLASSES || (n && !d_sizes) || n > MDISCR_MAX_ELEMS)
         return ctx_fail(ctx, MDISCR_EINVAL, "bad sizes_from_counts arguments");
-    if (n == 0) return MDISCR_OK;
     hipStream_t s = (hipStream_t)stream;
+    mdiscr_status st = ctx_begin(ctx, s);
+    if (st || n == 0) return st;
     const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 255) / 256, 8192);
     TimedLaunch tl;
     timing_begin(ctx, s, tl);

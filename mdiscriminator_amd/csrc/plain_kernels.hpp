@@ -462,7 +462,7 @@ __global__ __launch_bounds__(NW* kWave, MINW) void k_partition(
     const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t n,
     Discr dis, const uint32_t* __restrict__ prefix,
     const uint32_t* __restrict__ totals, uint32_t ld, uint32_t m,
-    uint32_t* __restrict__ user_counts) {
+    uint32_t* __restrict__ user_counts, uint64_t* __restrict__ offsets) {
     using G = Geom<NW, R, CT>;
     constexpr int MC = 1 << NBITS;
     constexpr int TILE = G::kTile;
@@ -536,6 +536,10 @@ __global__ __launch_bounds__(NW* kWave, MINW) void k_partition(
             } else if (t == t0) {
                 running = wave_excl_scan(tot, lane) + pre;
             }
+            if (offsets && t == 0) {  // the class starts (tile 0's running offsets)
+                if (lane < m) offsets[lane] = running;
+                if (lane == 0) offsets[m] = n;
+            }
             tile_offsets<NW, MC>(cursor, dpos, running, lane);
         }
         __syncthreads();
@@ -560,6 +564,274 @@ __global__ __launch_bounds__(NW* kWave, MINW) void k_partition(
                 const uint32_t nb = wbase + (uint32_t)TILE;
                 if ((uint64_t)(tbase + TILE) + TILE <= n) load_rounds<R, true>(in, v, nb, n, lane);
                 else load_rounds<R, false>(in, v, nb, n, lane);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Quad partition: the plain partition with 16-byte loads, for up to 16
+// classes and a 16-byte aligned input.  Lane l of wave w holds Q = R/4
+// quads; quad q holds elements wbase + 256q + 4l + j (j = 0..3), so each load
+// instruction reads 1 KiB of consecutive input and the wave's element order
+// (q, lane, j) is memory order.
+//
+// Rank inside a wave, one quad at a time and without ballots: each lane adds
+// its four classes into packed byte counters (class c in byte c & 7 of 64-bit
+// word c >> 3).  A DPP scan of those words over the wave leaves in byte c the
+// number of class-c elements in lower lanes, at most 63 * 4 = 252, so no byte
+// carries into the next (only lane 63's inclusive sum can reach 256, and it is
+// never used as such).  An element's rank in its (wave, quad, class) group is
+// that byte plus the class-c elements before it in its own quad.  The group
+// sizes go to LDS, and wave 0 turns them into LDS slot bases ordered by class,
+// then wave, then quad: within a class, stage order is memory order.
+//
+// OUT16: each class run is staged at base_c + 3c + pad_c with pad_c < 4 chosen
+// so that LDS slot and output index agree mod 4; the write-out reads four
+// slots per lane and stores them with one 16-byte store when all four belong
+// to one run, element by element otherwise.  Runs stay disjoint: run c-1 ends
+// at or before base_c + 3c.
+//
+// offsets (optional, device, m+1 entries): the class starts, written by the
+// workgroup of tile 0, and offsets[m] = n.
+template <int NBITS>
+struct QuadRank {
+    static_assert(NBITS <= 4, "quad ranking packs at most 16 classes");
+    static constexpr int kWords = NBITS <= 3 ? 1 : 2;       // 64-bit counter words
+    static constexpr int kScan = NBITS <= 2 ? 1 : 2 * kWords;  // 32-bit halves that can be non-zero
+
+    // byte c of a packed counter set
+    static __device__ __forceinline__ uint32_t byte_of(const uint64_t (&w)[2], uint32_t c) {
+        // both words are read first: a select between values, not addresses
+        const uint64_t w0 = w[0], w1 = w[1];
+        const uint64_t x = (NBITS > 3 && (c & 8u)) ? w1 : w0;
+        return (uint32_t)(x >> ((c & 7u) * 8u)) & 0xFFu;
+    }
+    static __device__ __forceinline__ void add(uint64_t (&w)[2], uint32_t c) {
+        const uint64_t inc = 1ull << ((c & 7u) * 8u);
+        if (NBITS <= 3) {
+            w[0] += inc;
+        } else {
+            const bool up = (c & 8u) != 0;
+            w[0] += up ? 0ull : inc;
+            w[1] += up ? inc : 0ull;
+        }
+    }
+    static __device__ __forceinline__ uint32_t half(const uint64_t (&w)[2], int i) {
+        return (uint32_t)(w[i >> 1] >> ((i & 1) * 32));
+    }
+    static __device__ __forceinline__ void set_half(uint64_t (&w)[2], int i, uint32_t x) {
+        const uint64_t keep = (i & 1) ? 0xFFFFFFFFull : 0xFFFFFFFF00000000ull;
+        w[i >> 1] = (w[i >> 1] & keep) | ((uint64_t)x << ((i & 1) * 32));
+    }
+};
+
+// Inclusive scans of K independent words across the wave, step by step in
+// lockstep so that each DPP move's source was written K instructions earlier
+// (the wait states a DPP read of a just-written register needs are filled
+// with the other words' work).
+template <int K>
+__device__ __forceinline__ void wave_incl_scan_dpp_n(uint32_t (&x)[K]) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) x[k] += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x[k], 0x111, 0xf, 0xf, false);
+#pragma unroll
+    for (int k = 0; k < K; ++k) x[k] += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x[k], 0x112, 0xf, 0xf, false);
+#pragma unroll
+    for (int k = 0; k < K; ++k) x[k] += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x[k], 0x114, 0xf, 0xf, false);
+#pragma unroll
+    for (int k = 0; k < K; ++k) x[k] += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x[k], 0x118, 0xf, 0xf, false);
+#pragma unroll
+    for (int k = 0; k < K; ++k) x[k] += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x[k], 0x142, 0xa, 0xf, false);
+#pragma unroll
+    for (int k = 0; k < K; ++k) x[k] += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x[k], 0x143, 0xc, 0xf, false);
+}
+
+// Phase 1 of the quad partition for one wave: the classes (ck) and the ranks
+// within their (wave, quad, class) group (rk) of Q quads, one byte per
+// element; group sizes to qb[q][c].  !FULL: elements at or past n (index
+// e_base + 256q + 4 lane + j) rank last as class m-1.
+template <int Q, int NBITS, bool FULL, class Discr>
+__device__ __forceinline__ void quad_rank(const u32x4 (&v)[Q], uint32_t (&ck)[Q], uint32_t (&rk)[Q],
+                                          uint32_t (*qb)[1 << NBITS], uint32_t e_base, uint32_t n,
+                                          uint32_t m, uint32_t lane, Discr dis) {
+    using QR = QuadRank<NBITS>;
+    constexpr int MC = 1 << NBITS;
+    constexpr int K = QR::kScan;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const uint32_t x[4] = {v[q].x, v[q].y, v[q].z, v[q].w};
+        uint64_t h[2] = {0ull, 0ull};
+        uint32_t c[4], lp[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            c[j] = dis(x[j]);
+            if (!FULL) c[j] = (e_base + q * (kWave * 4) + lane * 4 + j < n) ? c[j] : m - 1u;
+            lp[j] = j ? QR::byte_of(h, c[j]) : 0u;
+            QR::add(h, c[j]);
+        }
+        // Exclusive counts of lower lanes: scan, then subtract the own word.
+        uint32_t sc[K], own[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) own[k] = sc[k] = QR::half(h, k);
+        wave_incl_scan_dpp_n<K>(sc);
+        uint64_t ex[2] = {0ull, 0ull}, e63[2] = {0ull, 0ull}, h63[2] = {0ull, 0ull};
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const uint32_t e = sc[k] - own[k];
+            QR::set_half(ex, k, e);
+            QR::set_half(e63, k, (uint32_t)__builtin_amdgcn_readlane((int)e, kWave - 1));
+            QR::set_half(h63, k, (uint32_t)__builtin_amdgcn_readlane((int)own[k], kWave - 1));
+        }
+        uint32_t cw = 0, rw = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            cw |= c[j] << (8 * j);
+            rw |= (QR::byte_of(ex, c[j]) + lp[j]) << (8 * j);
+        }
+        // Materialised here, so the scan words die before the next quad.
+        asm volatile("" : "+v"(cw), "+v"(rw));
+        ck[q] = cw;
+        rk[q] = rw;
+        // Group size of class `lane`: lane 63's exclusive count plus its own.
+        if (lane < (uint32_t)MC) qb[q][lane] = QR::byte_of(e63, lane) + QR::byte_of(h63, lane);
+    }
+}
+
+template <int NW, int R, int NBITS, int WB, bool OUT16, int MINW, class Discr, bool SINGLE = false>
+__global__ __launch_bounds__(NW* kWave, MINW) void k_partition_quad(
+    const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t n, Discr dis,
+    const uint32_t* __restrict__ prefix, const uint32_t* __restrict__ totals, uint32_t ld,
+    uint32_t m, uint32_t* __restrict__ user_counts, uint64_t* __restrict__ offsets) {
+    constexpr int MC = 1 << NBITS;
+    constexpr int Q = R / 4;
+    constexpr int THREADS = NW * kWave;
+    constexpr int TILE = THREADS * R;
+    constexpr int SLOTS = OUT16 ? ((TILE + 3 * MC + 3) & ~3) : TILE;
+    static_assert(R % 4 == 0, "whole quads per lane");
+    __shared__ __attribute__((aligned(16))) uint32_t stage[SLOTS];
+    __shared__ uint32_t qbase[NW][Q][MC];  // group sizes, then LDS slot bases
+    __shared__ uint32_t dpos[MC];          // output index - LDS slot, per class
+    __shared__ uint32_t meta[MC];          // OUT16: run start | valid run length << 16
+
+    const uint32_t lane = lane_id();
+    const uint32_t wave = wave_id();
+    const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t tbase = t * (uint32_t)TILE;
+    const uint32_t wbase = tbase + wave * (uint32_t)(kWave * R);
+    const bool full = (uint64_t)tbase + TILE <= n;
+    const uint32_t valid = full ? (uint32_t)TILE : n - tbase;
+
+    uint32_t pre = 0, tot = 0;
+    if (!SINGLE && wave == 0 && lane < m) {
+        pre = prefix[(size_t)lane * ld + t];
+        tot = totals[lane];
+    }
+
+    // Loads: one per-lane pointer, quad q at the constant offset q KiB.
+    u32x4 v[Q];
+    if (full) {
+        const u32x4* lp = reinterpret_cast<const u32x4*>(in + wbase) + lane;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) v[q] = ld_stream(lp + q * kWave);
+    } else {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const uint32_t e = wbase + q * (kWave * 4) + lane * 4;
+            v[q].x = e < n ? in[e] : 0u;
+            v[q].y = e + 1 < n ? in[e + 1] : 0u;
+            v[q].z = e + 2 < n ? in[e + 2] : 0u;
+            v[q].w = e + 3 < n ? in[e + 3] : 0u;
+        }
+    }
+
+    // 1. Rank every element in its (wave, quad, class) group.  ck/rk hold the
+    // quad's four classes / ranks in bytes j.
+    uint32_t ck[Q], rk[Q];
+    if (full) quad_rank<Q, NBITS, true>(v, ck, rk, qbase[wave], wbase, n, m, lane, dis);
+    else quad_rank<Q, NBITS, false>(v, ck, rk, qbase[wave], wbase, n, m, lane, dis);
+    __syncthreads();
+
+    // 2. LDS slot bases per (wave, quad, class) and output offsets per class.
+    if (wave == 0) {
+        uint32_t g[NW * Q];
+        uint32_t ct = 0;
+        if (lane < (uint32_t)MC) {
+#pragma unroll
+            for (int i = 0; i < NW * Q; ++i) {
+                g[i] = qbase[i / Q][i % Q][lane];
+                const uint32_t x = g[i];
+                g[i] = ct;
+                ct += x;
+            }
+        }
+        uint32_t running;
+        if constexpr (SINGLE) {
+            running = wave_excl_scan(ct, lane);
+            if (user_counts && lane < m)
+                user_counts[lane] = ct - (lane == m - 1u ? (uint32_t)TILE - n : 0u);
+        } else {
+            running = wave_excl_scan(tot, lane) + pre;
+        }
+        if (offsets && t == 0) {
+            if (lane < m) offsets[lane] = running;
+            if (lane == 0) offsets[m] = n;
+        }
+        const uint32_t toff = wave_excl_scan(ct, lane);
+        uint32_t base = toff;
+        if (OUT16) {
+            const uint32_t a = (uint32_t)(reinterpret_cast<uintptr_t>(out) >> 2) & 3u;
+            base = toff + 3u * lane + ((running + a - toff - 3u * lane) & 3u);
+            const uint32_t rl = ct - ((lane == m - 1u) ? (uint32_t)TILE - valid : 0u);
+            if (lane < (uint32_t)MC) meta[lane] = base | (rl << 16);
+        }
+        if (lane < (uint32_t)MC) {
+#pragma unroll
+            for (int i = 0; i < NW * Q; ++i) qbase[i / Q][i % Q][lane] = g[i] + base;
+            dpos[lane] = running - base;
+        }
+    }
+    __syncthreads();
+
+    // 3. Stage the tile in LDS grouped by class.
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const uint32_t x[4] = {v[q].x, v[q].y, v[q].z, v[q].w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t c = (ck[q] >> (8 * j)) & 0xFFu;
+            const uint32_t r = (rk[q] >> (8 * j)) & 0xFFu;
+            stage[qbase[wave][q][c] + r] = x[j];
+        }
+    }
+    __syncthreads();
+
+    // 4. Class runs to global memory.
+    if constexpr (!OUT16) {
+        if (full) write_runs<R, THREADS, WB, true>(stage, dpos, out, valid, dis);
+        else write_runs<R, THREADS, WB, false>(stage, dpos, out, valid, dis);
+    } else {
+        constexpr int GROUPS = SLOTS / 4;
+        const u32x4* st4 = reinterpret_cast<const u32x4*>(stage);
+#pragma unroll
+        for (int k = 0; k < (GROUPS + THREADS - 1) / THREADS; ++k) {
+            const uint32_t gi = k * THREADS + threadIdx.x;
+            if ((k + 1) * THREADS > GROUPS && gi >= (uint32_t)GROUPS) break;
+            const u32x4 x = st4[gi];
+            const uint32_t s0 = 4u * gi;
+            const uint32_t d0 = dis(x.x), d3 = dis(x.w);
+            const uint32_t m0 = meta[d0], m3 = meta[d3];
+            const bool in0 = s0 - (m0 & 0xFFFFu) < (m0 >> 16);
+            const bool in3 = s0 + 3u - (m3 & 0xFFFFu) < (m3 >> 16);
+            if (in0 && in3 && d0 == d3) {
+                st_stream(x, reinterpret_cast<u32x4*>(out + (dpos[d0] + s0)));
+            } else {
+                const uint32_t xs[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const uint32_t d = dis(xs[j]);
+                    const uint32_t mt = meta[d];
+                    if (s0 + j - (mt & 0xFFFFu) < (mt >> 16)) st_stream(xs[j], out + (dpos[d] + s0 + j));
+                }
             }
         }
     }

@@ -177,26 +177,6 @@ struct SegState {
     hipEvent_t staged_evt = nullptr;  // last upload out of host_desc
 };
 
-// Grow-only device buffer (the old one may still be read by queued work).
-mdiscr_status grow_buffer(mdiscr_ctx* ctx, void** buf, size_t* have, size_t need) {
-    if (need <= *have) return MDISCR_OK;
-    if (*buf) {
-        mdiscr_status st = ctx_hip(ctx, hipDeviceSynchronize(), "hipDeviceSynchronize");
-        if (st) return st;
-        (void)hipFree(*buf);
-        *buf = nullptr;
-        *have = 0;
-    }
-    hipError_t e = hipMalloc(buf, need);
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
-        *buf = nullptr;
-        return ctx_fail(ctx, MDISCR_ENOMEM, "%zu-byte buffer: %s", need, hipGetErrorString(e));
-    }
-    *have = need;
-    return MDISCR_OK;
-}
-
 SegState* seg_state(mdiscr_ctx* ctx) {
     if (!ctx->extra) ctx->extra = new SegState();
     return static_cast<SegState*>(ctx->extra);
@@ -481,7 +461,36 @@ __global__ void k_seg_new_sizes(const uint64_t* __restrict__ off, uint32_t nseg,
     }
 }
 
+// Pinned descriptor staging for at least nreq batch requests (mapped, so
+// the planner reads it directly).  The caller has waited for the last upload
+// out of the old buffer.
+mdiscr_status stage_capacity(mdiscr_ctx* ctx, SegState* ss, uint32_t nreq) {
+    const size_t dbytes = (size_t)nreq * sizeof(SegDesc);
+    if (dbytes <= ss->host_desc_bytes) return MDISCR_OK;
+    if (ss->host_desc) (void)hipHostFree(ss->host_desc);
+    ss->host_desc = nullptr;
+    ss->host_desc_dev = nullptr;
+    ss->host_desc_bytes = 0;
+    hipError_t e = hipHostMalloc(&ss->host_desc, dbytes * 2, hipHostMallocMapped);
+    if (e == hipSuccess) e = hipHostGetDevicePointer(&ss->host_desc_dev, ss->host_desc, 0);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return ctx_fail(ctx, MDISCR_ENOMEM, "pinned staging: %s", hipGetErrorString(e));
+    }
+    ss->host_desc_bytes = dbytes * 2;
+    return MDISCR_OK;
+}
+
 }  // namespace
+
+mdiscr_status ctx_reserve_batch(mdiscr_ctx* ctx, uint64_t max_elems, uint32_t max_reqs) {
+    mdiscr_status st = ctx_hip(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    if (st) return st;
+    SegWorkspace w;
+    st = seg_workspace(ctx, max_elems, max_reqs, MDISCR_MAX_CLASSES, w);
+    if (st) return st;
+    return stage_capacity(ctx, seg_state(ctx), max_reqs);
+}
 
 void ctx_release_extra(mdiscr_ctx* ctx) {
     if (ctx->small) (void)hipFree(ctx->small);
@@ -520,10 +529,10 @@ mdiscr_status mdiscr_partition_segmented(mdiscr_ctx* ctx, const uint32_t* d_in, 
             if (h_seg_off[s + 1] < h_seg_off[s])
                 return ctx_fail(ctx, MDISCR_EINVAL, "offsets must not decrease");
     }
-    st = ctx_hip(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    hipStream_t s = (hipStream_t)stream;
+    st = ctx_begin(ctx, s);
     if (st) return st;
     if (nseg == 0) return MDISCR_OK;
-    hipStream_t s = (hipStream_t)stream;
     const int cfg = pick_config(ctx, d_in, d_out, n, d_seg_off, nseg, d, s, &st);
     if (st) return st;
     return run_segmented(ctx, d_in, d_out, n, d_seg_off, nseg, d, d_seg_counts, s, cfg, true);
@@ -540,10 +549,10 @@ mdiscr_status mdiscr_segm_flags(mdiscr_ctx* ctx, uint64_t n, const uint32_t* d_i
         return ctx_fail(ctx, MDISCR_EINVAL, "null pointer");
     if (n && ranges_overlap(d_in, d_out, n * 4))
         return ctx_fail(ctx, MDISCR_EINVAL, "output overlaps input");
-    st = ctx_hip(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    hipStream_t s = (hipStream_t)stream;
+    st = ctx_begin(ctx, s);
     if (st) return st;
     if (n == 0) return MDISCR_OK;
-    hipStream_t s = (hipStream_t)stream;
     SegState* ss = seg_state(ctx);
     // flag workspace: bad-flag counter | block counts (nb+1) | offsets (n+1)
     const uint32_t nb = (uint32_t)((n + kFlagBlock - 1) / kFlagBlock);
@@ -551,7 +560,7 @@ mdiscr_status mdiscr_segm_flags(mdiscr_ctx* ctx, uint64_t n, const uint32_t* d_i
     const size_t o_bc = 256;
     const size_t o_off = o_bc + (((size_t)nbl * 4 + 255) & ~(size_t)255);
     const size_t need = o_off + (n + 1) * 8;
-    st = grow_buffer(ctx, &ss->flag_ws, &ss->flag_ws_bytes, need);
+    st = ctx_grow(ctx, &ss->flag_ws, &ss->flag_ws_bytes, need);
     if (st) return st;
     char* b = static_cast<char*>(ss->flag_ws);
     uint32_t* bad = reinterpret_cast<uint32_t*>(b);
@@ -578,7 +587,7 @@ mdiscr_status mdiscr_segm_flags(mdiscr_ctx* ctx, uint64_t n, const uint32_t* d_i
     st = ctx_hip(ctx, hipStreamSynchronize(s), "hipStreamSynchronize");
     if (st) return st;
     if (nbad) return ctx_fail(ctx, MDISCR_EINVAL, "segment_sizes do not tile [0,n)");
-    st = grow_buffer(ctx, &ss->cnt_ws, &ss->cnt_ws_bytes, (size_t)nseg * d.m * 4 + 4);
+    st = ctx_grow(ctx, &ss->cnt_ws, &ss->cnt_ws_bytes, (size_t)nseg * d.m * 4 + 4);
     if (st) return st;
     uint32_t* seg_counts = static_cast<uint32_t*>(ss->cnt_ws);
     const int cfg = pick_config(ctx, d_in, d_out, n, off, nseg, d, s, &st);
@@ -615,9 +624,9 @@ mdiscr_status mdiscr_partition_batch(mdiscr_ctx* ctx, const mdiscr_request* reqs
         if (r.d.kind != MDISCR_DISCR_MASK && (r.d.m & (r.d.m - 1))) all_pow2 = false;
     }
     if (total > MDISCR_MAX_ELEMS) return ctx_fail(ctx, MDISCR_EINVAL, "batch too large");
-    mdiscr_status st = ctx_hip(ctx, hipSetDevice(ctx->device), "hipSetDevice");
-    if (st || nreq == 0) return st;
     hipStream_t s = (hipStream_t)stream;
+    mdiscr_status st = ctx_begin(ctx, s);
+    if (st || nreq == 0) return st;
     SegState* ss = seg_state(ctx);
     // Host descriptors in pinned staging; the previous batch's upload must
     // have left the staging buffer before it is rewritten.
@@ -629,20 +638,8 @@ mdiscr_status mdiscr_partition_batch(mdiscr_ctx* ctx, const mdiscr_request* reqs
                      "hipEventCreate");
         if (st) return st;
     }
-    const size_t dbytes = (size_t)nreq * sizeof(SegDesc);
-    if (dbytes > ss->host_desc_bytes) {
-        if (ss->host_desc) (void)hipHostFree(ss->host_desc);
-        ss->host_desc = nullptr;
-        ss->host_desc_dev = nullptr;
-        ss->host_desc_bytes = 0;
-        hipError_t e = hipHostMalloc(&ss->host_desc, dbytes * 2, hipHostMallocMapped);
-        if (e == hipSuccess) e = hipHostGetDevicePointer(&ss->host_desc_dev, ss->host_desc, 0);
-        if (e != hipSuccess) {
-            (void)hipGetLastError();
-            return ctx_fail(ctx, MDISCR_ENOMEM, "pinned staging: %s", hipGetErrorString(e));
-        }
-        ss->host_desc_bytes = dbytes * 2;
-    }
+    st = stage_capacity(ctx, ss, nreq);
+    if (st) return st;
     SegDesc* hd = static_cast<SegDesc*>(ss->host_desc);
     for (uint32_t i = 0; i < nreq; ++i) {
         hd[i].in = reqs[i].d_in;

@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "../../include/mdiscr.h"
+#include "api_internal.hpp"
 
 namespace {
 
@@ -68,6 +69,7 @@ struct mdiscr_server {
     std::map<size_t, std::vector<void*>> free_blocks;
     size_t held = 0, in_use = 0, peak = 0;
     uint64_t next_ticket = 1, batches = 0, batched_reqs = 0, completed = 0, elems = 0;
+    uint64_t cancelled = 0;
     std::vector<double> latencies;
     bool stop = false;
     std::thread worker;
@@ -173,7 +175,9 @@ struct mdiscr_server {
                 admitted.push_back(queued.front());
                 queued.pop_front();
             }
-            // 3. one batch per free slot
+            // 3. one batch per free slot, launched with mu released so that
+            // submit, wait and release never block on a launch.  The slot is
+            // claimed (busy) first; only this thread touches slots.
             for (auto& sl : slots) {
                 if (sl.busy || admitted.empty()) continue;
                 batch.clear();
@@ -189,21 +193,26 @@ struct mdiscr_server {
                     belems += r.n;
                     admitted.pop_front();
                 }
+                sl.busy = true;
+                lk.unlock();
                 mdiscr_status st = mdiscr_partition_batch(sl.ctx, batch.data(),
                                                           (uint32_t)batch.size(), sl.stream);
                 if (st == MDISCR_OK)
                     st = hipEventRecord(sl.done, sl.stream) == hipSuccess ? MDISCR_OK
                                                                           : MDISCR_EDEVICE;
+                lk.lock();
                 if (st != MDISCR_OK) {
                     for (uint64_t t : sl.tickets) {
-                        reqs[t].state = 4;
-                        reqs[t].err = st;
+                        auto it = reqs.find(t);
+                        if (it == reqs.end()) continue;
+                        it->second.state = 4;
+                        it->second.err = st;
                     }
                     sl.tickets.clear();
+                    sl.busy = false;
                     cv_done.notify_all();
                     continue;
                 }
-                sl.busy = true;
                 ++batches;
                 batched_reqs += batch.size();
                 progressed = true;
@@ -252,6 +261,13 @@ mdiscr_status mdiscr_server_create(mdiscr_server** out, const mdiscr_server_conf
             (void)hipGetLastError();
             mdiscr_server_destroy(s);
             return MDISCR_EDEVICE;
+        }
+        // Every batch fits the slot's workspace and staging from the start, so
+        // no launch grows a buffer while the other slots' batches run.
+        const uint64_t max_elems = std::min<uint64_t>(cfg->max_batch_elems, MDISCR_MAX_ELEMS);
+        if (ctx_reserve_batch(sl.ctx, max_elems, cfg->max_batch_requests) != MDISCR_OK) {
+            mdiscr_server_destroy(s);
+            return MDISCR_ENOMEM;
         }
     }
     for (uint64_t r = 0; r + mdiscr_server::kChunk <= std::min(cfg->reserve_bytes, cfg->budget_bytes);
@@ -331,14 +347,43 @@ mdiscr_status mdiscr_server_wait(mdiscr_server* s, uint64_t ticket, uint32_t** d
                                  uint32_t* h_counts, double* latency_s) {
     if (!s) return MDISCR_EINVAL;
     std::unique_lock<std::mutex> lk(s->mu);
-    auto it = s->reqs.find(ticket);
-    if (it == s->reqs.end()) return MDISCR_EINVAL;
-    s->cv_done.wait(lk, [&] { return it->second.state >= 3; });
-    Request& r = it->second;
+    // The entry is looked up again after every wake: other threads insert
+    // (rehashing the map) and a cancel may remove it meanwhile.
+    Request* rp = nullptr;
+    s->cv_done.wait(lk, [&] {
+        auto it = s->reqs.find(ticket);
+        rp = it == s->reqs.end() ? nullptr : &it->second;
+        return !rp || rp->state >= 3;
+    });
+    if (!rp) return MDISCR_EINVAL;  // unknown, or cancelled while waiting
+    Request& r = *rp;
     if (r.state == 4) return r.err ? r.err : MDISCR_EDEVICE;
     if (d_out) *d_out = r.d_out;
     if (h_counts) std::copy(r.h_counts, r.h_counts + r.d.m, h_counts);
     if (latency_s) *latency_s = std::chrono::duration<double>(r.t_done - r.t_submit).count();
+    return MDISCR_OK;
+}
+
+mdiscr_status mdiscr_server_cancel(mdiscr_server* s, uint64_t ticket) {
+    if (!s) return MDISCR_EINVAL;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        auto it = s->reqs.find(ticket);
+        if (it == s->reqs.end()) return MDISCR_EINVAL;
+        Request& r = it->second;
+        if (r.state == 0) {
+            s->queued.erase(std::find(s->queued.begin(), s->queued.end(), ticket));
+        } else if (r.state == 1) {
+            s->admitted.erase(std::find(s->admitted.begin(), s->admitted.end(), ticket));
+            s->free_block(r.d_out, r.out_bytes);
+        } else {
+            return MDISCR_EBUSY;  // on the GPU or finished: wait and release instead
+        }
+        s->reqs.erase(it);
+        ++s->cancelled;
+    }
+    s->cv_done.notify_all();  // waiters on the ticket return
+    s->cv_work.notify_one();  // freed budget may admit the next request
     return MDISCR_OK;
 }
 

@@ -196,7 +196,7 @@ mdiscr_status run_soa(mdiscr_ctx* ctx, SoaCols& cols, uint64_t n, const ClassSpe
                       uint32_t* d_counts, void* stream) {
     hipStream_t s = (hipStream_t)stream;
     const uint32_t m = cs.m();
-    mdiscr_status st = ctx_hip(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    mdiscr_status st = ctx_begin(ctx, s);
     if (st) return st;
     if (n == 0) {
         if (d_counts) {

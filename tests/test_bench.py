@@ -33,6 +33,12 @@ def test_bench_line_is_complete():
         assert d["soa_records_2p26"][m]["partition_frac"] > 0
     for name, line in d["general"].items():
         assert line["partition_frac"] and line["partition_frac"] > 0, name
+    for m in ("m2", "m4", "m16", "m32", "m64", "mod7"):
+        assert 0 < d["plain_across_m"][m]["partition_frac"] < 1, m
+    for k in ("n100", "n4096"):
+        assert d["small_calls"][k]["us_per_call"] > 0
+    assert 0 < d["hbm_frac_whole_path_12B"] < 1
+    assert "traffic_source" in rf
     assert d["batched_segments"]["identical"] is True
     assert d["serving"]["identical_spot_checks"] is True
 
@@ -43,3 +49,19 @@ def test_bench_refuses_several_gpus():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+def test_cpu_baseline_script_quick():
+    """oracle/cpu_baseline.py on small samples (CPU): every configuration's
+    line, the host description and the reference method."""
+    env = dict(os.environ, OMP_PROC_BIND="close", OMP_NUM_THREADS="2")
+    r = subprocess.run([sys.executable, "-m", "oracle.cpu_baseline", "--quick", "--runs", "3"],
+                       cwd=ROOT, env=env, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    d = json.loads(r.stdout.strip().splitlines()[-1])
+    for k in ("config1_seq", "config2_seq", "config2_openmp", "config4_seq"):
+        assert d[k]["value"] > 0 and len(d[k]["runs_ms"]) == 3, k
+    assert d["config2_openmp"]["cores"] == 2
+    for m in ("m16", "m32", "m64"):
+        assert d["config3_seq"][m]["value"] > 0
+    assert d["host"]["nproc"] >= 1 and "run.py" in d["method"]
