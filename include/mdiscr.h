@@ -105,6 +105,26 @@ mdiscr_status mdiscr_partition_u32_ex(mdiscr_ctx* ctx, const uint32_t* d_in,
                                       uint64_t* offsets, uint32_t flags,
                                       void* stream);
 
+/* Bytes of the fixed device scratch of mdiscr_partition_u32_noalloc (one per
+ * GPU, held until the process ends). */
+#define MDISCR_NOALLOC_SCRATCH_BYTES (8u << 20)
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (serving loops and harnesses that forbid hipMalloc): the plain
+ * partition of mdiscr_partition_u32_ex (device counts and offsets, either
+ * may be NULL) with its temporaries in a fixed static device buffer of
+ * MDISCR_NOALLOC_SCRATCH_BYTES per GPU instead of the context workspace.  The
+ * call allocates and frees nothing (its kernels need (2*ld + 1) * m * 4 bytes,
+ * ld = ceil(n / 4096) rounded up to a multiple of 4: n up to 2^29 at m = 8,
+ * about 2^26 at m = 64); larger n and m are refused with MDISCR_ENOMEM.  Every
+ * call on one GPU shares that GPU's buffer: calls that overlap on one GPU (on
+ * different streams or host threads) race and must be serialised by the
+ * caller.  Streams under graph capture are refused like everywhere else. */
+mdiscr_status mdiscr_partition_u32_noalloc(mdiscr_ctx* ctx, const uint32_t* d_in,
+                                           uint32_t* d_out, uint64_t n,
+                                           mdiscr_discr d, uint32_t* d_counts,
+                                           uint64_t* d_offsets, void* stream);
+
 /* Reference-shaped synchronous call: like mdiscr<DISCR>(n, num_hwd_thds,
  * d_in, d_out), the class counts are returned in host memory h_counts (m
  * entries) after the device work has finished.  num_hwd_thds is accepted

@@ -66,6 +66,7 @@ SIGNATURES = {
     "mdiscr_partition_u32": (_int, [_vp, _vp, _vp, _u64, MdiscrDiscr, _vp, _vp]),
     "mdiscr_u32": (_int, [_vp, _u64, _u32, _vp, _vp, MdiscrDiscr, _vp]),
     "mdiscr_partition_u32_ex": (_int, [_vp, _vp, _vp, _u64, MdiscrDiscr, _vp, _vp, _u32, _vp]),
+    "mdiscr_partition_u32_noalloc": (_int, [_vp, _vp, _vp, _u64, MdiscrDiscr, _vp, _vp, _vp]),
     "mdiscr_partition_soa": (_int, [_vp, _u32, _vp, _vp, _u32, _vp, _u64, MdiscrDiscr, _vp, _vp]),
     "mdiscr_partition_arrays": (_int, [_vp, _vp, _vp, _u64, _u32, MdiscrDiscr, _vp, _vp]),
     "mdiscr_filter_u32": (_int, [_vp, _vp, _vp, _u64, MdiscrPred, _vp, _vp]),

@@ -32,6 +32,7 @@ struct mdiscr_ctx {
     std::vector<TimedLaunch> pending;
     std::map<std::string, std::pair<double, uint64_t>> timing_acc;
     void* extra = nullptr;     // segmented-path planning buffers
+    void* scratch = nullptr;   // this device's no-allocation scratch (device address)
     char err[256] = {0};
 };
 

@@ -115,6 +115,36 @@ void timing_end(mdiscr_ctx* ctx, hipStream_t s, TimedLaunch& t, const char* name
     ctx->pending.push_back(t);
 }
 
+// ---------------------------------------------------------------------------
+// Fixed device scratch of the no-allocation entry point: per GPU one copy of
+// `scratch`, resolved once per device at context creation.  Host code uses
+// only the resolved addresses (the array's name is a host-side stand-in).
+
+static __device__ __align__(256) uint32_t scratch[MDISCR_NOALLOC_SCRATCH_BYTES / 4];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxDevices] = {};
+
+// The device address of `scratch` on `device` (current), resolved on first use.
+mdiscr_status scratch_address(int device, void** out) {
+    if (device < 0 || device >= kMaxDevices) return MDISCR_EUNSUPPORTED;
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return MDISCR_EDEVICE;
+        }
+        g_scratch[device] = p;
+    }
+    *out = g_scratch[device];
+    return MDISCR_OK;
+}
+}  // namespace
+
 extern "C" {
 
 mdiscr_status mdiscr_ctx_create(mdiscr_ctx** out, int device) {
@@ -135,6 +165,12 @@ mdiscr_status mdiscr_ctx_create(mdiscr_ctx** out, int device) {
     ctx->device = device;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cu = prop.multiProcessorCount;
+    // Set-up of the no-allocation entry point's scratch on this device.
+    const mdiscr_status st = scratch_address(device, &ctx->scratch);
+    if (st != MDISCR_OK) {
+        delete ctx;
+        return st;
+    }
     *out = ctx;
     return MDISCR_OK;
 }
@@ -595,6 +631,32 @@ mdiscr_status mdiscr_u32(mdiscr_ctx* ctx, uint64_t n, uint32_t num_hwd_thds,
     if (!ctx || !h_counts) return ctx_fail(ctx, MDISCR_EINVAL, "null argument");
     return mdiscr_partition_u32_ex(ctx, d_in, d_out, n, d, h_counts, nullptr, MDISCR_OUT_HOST,
                                    nullptr);
+}
+
+mdiscr_status mdiscr_partition_u32_noalloc(mdiscr_ctx* ctx, const uint32_t* d_in,
+                                           uint32_t* d_out, uint64_t n, mdiscr_discr d,
+                                           uint32_t* d_counts, uint64_t* d_offsets,
+                                           void* stream) {
+    if (!ctx) return MDISCR_EINVAL;
+    mdiscr_status st = check_discr(ctx, d);
+    if (st) return st;
+    if (n <= MDISCR_MAX_ELEMS && plain_ws_bytes(n, d.m) > MDISCR_NOALLOC_SCRATCH_BYTES)
+        return ctx_fail(ctx, MDISCR_ENOMEM,
+                        "n=%llu with m=%u needs %zu bytes of temporaries; the no-allocation "
+                        "scratch holds %u",
+                        (unsigned long long)n, d.m, plain_ws_bytes(n, d.m),
+                        (unsigned)MDISCR_NOALLOC_SCRATCH_BYTES);
+    hipStream_t s = (hipStream_t)stream;
+    bool done = true;
+    st = plain_check(ctx, d_in, d_out, n, d.m, d_counts, d_offsets, s, &done);
+    if (st || done) return st;
+    // No timing records: they would create events.
+    const bool timing = ctx->timing;
+    ctx->timing = false;
+    st = plain_run(ctx, d_in, d_out, n, d, d_counts, d_offsets, s,
+                   static_cast<uint32_t*>(ctx->scratch));
+    ctx->timing = timing;
+    return st;
 }
 
 mdiscr_status mdiscr_sizes_from_counts(mdiscr_ctx* ctx, const uint32_t* d_counts, uint32_t m,
