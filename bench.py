@@ -38,22 +38,23 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-# The sources the plain partition kernel is compiled from, with the build
-# flags: the HBM traffic in profiles/ was counted for one version of them.
+# The sources the plain partition kernel is compiled from: the HBM traffic in
+# profiles/ was counted for one version of them and of the compiler flags.
 KERNEL_SOURCES = ["mdiscriminator_amd/csrc/mdiscr_api.hip",
                   "mdiscriminator_amd/csrc/plain_kernels.hpp",
                   "mdiscriminator_amd/csrc/device_common.hpp",
                   "mdiscriminator_amd/csrc/divmagic.h",
-                  "mdiscriminator_amd/csrc/api_internal.hpp",
-                  "mdiscriminator_amd/build.py"]
+                  "mdiscriminator_amd/csrc/api_internal.hpp"]
 
 
 def kernel_source_hash():
     import hashlib
+    from mdiscriminator_amd import build
     h = hashlib.sha256()
     for f in KERNEL_SOURCES:
         with open(os.path.join(ROOT, f), "rb") as fh:
             h.update(fh.read())
+    h.update(repr(build.HIPCC_FLAGS).encode())
     return h.hexdigest()[:16]
 
 

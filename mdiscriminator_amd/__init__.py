@@ -277,6 +277,11 @@ class Context:
         nseg = d_seg_off.numel() - 1
         hp = None
         if h_seg_off is not None:
+            import numpy as np
+            h_seg_off = np.ascontiguousarray(h_seg_off, dtype=np.uint64)
+            if h_seg_off.shape != (nseg + 1,):
+                raise MdiscrError(_lib.EINVAL, f"h_seg_off has {h_seg_off.size} entries, "
+                                               f"the call needs {nseg + 1}")
             hp = h_seg_off.ctypes.data_as(ctypes.c_void_p)
         n = d_in.numel()
         st = self.lib.mdiscr_partition_segmented(self.h, _ptr(d_in, 4 * n), _ptr(d_out, 4 * n), n,

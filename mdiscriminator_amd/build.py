@@ -17,7 +17,7 @@ SOURCES = [
     "arrays.hip",
     "segmented.hip",
     "serve.cpp",
-]
+] + [f"seg_cfg_b{b}.hip" for b in range(7)]  # segmented candidates, one unit per class-bit count
 
 HIPCC_FLAGS = [
     "-O3",

@@ -19,119 +19,11 @@
 #include <tuple>
 
 #include "api_internal.hpp"
-#include "segmented_kernels.hpp"
+#include "segmented_launch.hpp"
 
 using namespace mdiscr;
 
 namespace {
-
-constexpr int kPlanThreads = 1024;
-constexpr uint32_t kExpandGrid = 1024;  // workgroups writing the T items of long segments
-constexpr uint32_t kExpandInPlan = kPlanExpandMax;  // the planner expands them itself
-
-struct SegConfig {
-    int nw, r;
-    bool staged;
-    const char* name;
-};
-
-// Candidates: block size {256,512,1024} x elements per thread {4..32} x
-// staged/direct scatter.  Entry 0 is the default and the only one built for
-// more than 16 classes or non-power-of-two m.
-constexpr SegConfig kConfigs[] = {
-    {4, 16, true, "b256_e16_staged"}, {4, 16, false, "b256_e16_direct"},
-    {8, 8, true, "b512_e8_staged"},   {8, 8, false, "b512_e8_direct"},
-    {4, 8, true, "b256_e8_staged"},   {8, 16, true, "b512_e16_staged"},
-    {16, 4, true, "b1024_e4_staged"}, {4, 32, true, "b256_e32_staged"},
-};
-constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
-constexpr uint32_t kMinTile = 1024;  // smallest NW*64*R among the candidates
-
-struct SegArgs {
-    OffSrc os{};                      // segments by offsets (descriptors built on the device)
-    const SegDesc* staged = nullptr;  // batch descriptors in mapped pinned memory
-    hipEvent_t staged_read = nullptr;  // recorded once the planner has read them
-    hipError_t* staged_err = nullptr;  // where that record's status goes
-    const SegDesc* desc;
-    SegItem* items;
-    SegLong* longs;
-    SegPlan* plan;
-    uint32_t* counts;
-    uint32_t* prefix;   // the scan of counts
-    uint32_t ld;
-    uint32_t max_rows;  // bound on the plan's row count
-    uint32_t nseg;
-    uint32_t mrows;
-    uint32_t grid_cap;  // no more workgroups than the queue can hold
-    uint32_t num_cu;
-};
-
-// Workgroups of `kernel` resident on one CU (the occupancy API), at least 1.
-template <typename K>
-int resident_per_cu(K kernel, int threads) {
-    int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, threads, 0) != hipSuccess ||
-        nb < 1)
-        nb = 1;
-    return nb;
-}
-
-#ifndef MDISCR_SEG_OVERSUB
-#define MDISCR_SEG_OVERSUB 16
-#endif
-// The queue-walking grid: MDISCR_SEG_OVERSUB times what fits at once, so
-// each workgroup takes a few items and finished workgroups are replaced by
-// the hardware (a grid of exactly the resident capacity leaves the last
-// items' imbalance as a tail), and never more workgroups than items.
-inline uint32_t persistent_grid(int per_cu, const SegArgs& a) {
-    uint64_t g = (uint64_t)per_cu * a.num_cu * MDISCR_SEG_OVERSUB;
-    if (g > a.grid_cap) g = a.grid_cap;
-    return (uint32_t)(g ? g : 1);
-}
-
-template <int NW, int R, int NBITS, bool MASK, bool STAGED>
-void launch_cfg(const SegArgs& a, hipStream_t s) {
-    const bool expand_here = a.nseg <= kExpandInPlan;
-    hipLaunchKernelGGL((k_seg_plan<kPlanThreads, NW, R>), dim3(1), dim3(kPlanThreads), 0, s,
-                       a.staged, a.os, const_cast<SegDesc*>(a.desc), a.nseg, a.items, a.longs,
-                       a.plan, a.counts, a.ld, a.mrows, expand_here ? 1u : 0u);
-    if (a.staged_read) *a.staged_err = hipEventRecord(a.staged_read, s);
-    if (!expand_here) {
-        const uint32_t eg = a.nseg < kExpandGrid ? (a.nseg ? a.nseg : 1u) : kExpandGrid;
-        hipLaunchKernelGGL((k_seg_expand<256>), dim3(eg), dim3(256), 0, s, a.longs, a.plan,
-                           a.items, a.os, const_cast<SegDesc*>(a.desc), a.nseg);
-    }
-    static const int count_per_cu = resident_per_cu(k_seg_count<NW, R, NBITS, MASK>, NW * kWave);
-    static const int part_per_cu =
-        resident_per_cu(k_seg_partition<NW, R, NBITS, MASK, STAGED>, NW * kWave);
-    hipLaunchKernelGGL((k_seg_count<NW, R, NBITS, MASK>), dim3(persistent_grid(count_per_cu, a)),
-                       dim3(NW * kWave), 0, s, a.desc, a.items, a.plan, a.counts, a.ld);
-    launch_tile_scan(a.counts, a.prefix, a.max_rows + 1u, a.ld, a.mrows, nullptr, nullptr,
-                     &a.plan->n_rows, s);
-    hipLaunchKernelGGL((k_seg_partition<NW, R, NBITS, MASK, STAGED>),
-                       dim3(persistent_grid(part_per_cu, a)), dim3(NW * kWave), 0, s, a.desc,
-                       a.items, a.plan, a.prefix, a.ld);
-}
-
-template <int NBITS, bool MASK>
-bool launch_nbits(int cfg, const SegArgs& a, hipStream_t s) {
-    if (cfg == 0) {
-        launch_cfg<4, 16, NBITS, MASK, true>(a, s);
-        return true;
-    }
-    if constexpr (MASK && NBITS <= 4) {
-        switch (cfg) {
-            case 1: launch_cfg<4, 16, NBITS, MASK, false>(a, s); return true;
-            case 2: launch_cfg<8, 8, NBITS, MASK, true>(a, s); return true;
-            case 3: launch_cfg<8, 8, NBITS, MASK, false>(a, s); return true;
-            case 4: launch_cfg<4, 8, NBITS, MASK, true>(a, s); return true;
-            case 5: launch_cfg<8, 16, NBITS, MASK, true>(a, s); return true;
-            case 6: launch_cfg<16, 4, NBITS, MASK, true>(a, s); return true;
-            case 7: launch_cfg<4, 32, NBITS, MASK, true>(a, s); return true;
-        }
-    }
-    return false;
-}
 
 template <bool MASK>
 bool launch_any(uint32_t nbits, int cfg, const SegArgs& a, hipStream_t s) {
@@ -147,9 +39,7 @@ bool launch_any(uint32_t nbits, int cfg, const SegArgs& a, hipStream_t s) {
     return false;
 }
 
-bool config_available(bool mask, uint32_t nbits, int cfg) {
-    return cfg == 0 || (mask && nbits <= 4 && cfg < kNumConfigs);
-}
+bool config_available(int cfg) { return cfg >= 0 && cfg < kNumConfigs; }
 
 // ---- per-context state: tuning table and captured graphs ------------------
 
@@ -182,13 +72,15 @@ SegState* seg_state(mdiscr_ctx* ctx) {
     return static_cast<SegState*>(ctx->extra);
 }
 
-std::string bucket_key(bool mask, uint32_t nbits, uint64_t n, uint64_t nseg) {
+// The tuning bucket of a call: element type, discriminator kind, class count
+// and the sizes rounded down to powers of two.
+std::string bucket_key(bool mask, uint32_t m, uint64_t n, uint64_t nseg) {
     int lg = 0;
     while ((1ull << (lg + 1)) <= n) ++lg;
     int ls = 0;
     while ((1ull << (ls + 1)) <= nseg) ++ls;
     char buf[96];
-    snprintf(buf, sizeof(buf), "u32 %s b%u n%d s%d", mask ? "mask" : "mod", nbits, lg, ls);
+    snprintf(buf, sizeof(buf), "u32 %s m%u n%d s%d", mask ? "mask" : "mod", m, lg, ls);
     return buf;
 }
 
@@ -215,7 +107,8 @@ void tune_save(SegState* st) {
     if (!path) return;
     FILE* f = fopen(path, "w");
     if (!f) return;
-    for (auto& kv : st->tuned) fprintf(f, "%s=%d\n", kv.first.c_str(), kv.second);
+    for (auto& kv : st->tuned)
+        fprintf(f, "%s=%d %s\n", kv.first.c_str(), kv.second, kConfigs[kv.second].name);
     fclose(f);
 }
 
@@ -287,8 +180,9 @@ mdiscr_status run_segmented(mdiscr_ctx* ctx, const uint32_t* d_in, uint32_t* d_o
     a.nseg = (uint32_t)nseg;
     a.mrows = d.m;
     a.grid_cap = grid_cap(2 * nseg + n / kMinTile + 2);
+    a.item_cap = a.grid_cap;
     a.num_cu = (uint32_t)ctx->num_cu;
-    a.os = OffSrc{d_seg_off, d_in, d_out, d_seg_counts, d.m, mdiscr_divmagic_make(d.m)};
+    a.os = OffSrc{d_seg_off, d_in, d_out, d_seg_counts, d.m, mdiscr_divmagic_make(d.m), n};
 
     auto issue = [&](hipStream_t q) -> bool {
         return mask ? launch_any<true>(nbits, cfg, a, q) : launch_any<false>(nbits, cfg, a, q);
@@ -345,14 +239,12 @@ int pick_config(mdiscr_ctx* ctx, const uint32_t* d_in, uint32_t* d_out, uint64_t
                 mdiscr_status* err) {
     *err = MDISCR_OK;
     const bool mask = d.kind == MDISCR_DISCR_MASK || (d.m & (d.m - 1)) == 0;
-    const uint32_t nbits = class_bits(d.m);
-    if (!(mask && nbits <= 4)) return 0;
-    if (const int c = forced_config(); c >= 0 && config_available(mask, nbits, c)) return c;
+    if (const int c = forced_config(); c >= 0 && config_available(c)) return c;
     const char* at = getenv("MDISCR_AUTOTUNE");
     if (at && at[0] == '0') return 0;
     SegState* ss = seg_state(ctx);
     tune_load(ss);
-    const std::string key = bucket_key(mask, nbits, n, nseg);
+    const std::string key = bucket_key(mask, d.m, n, nseg);
     auto it = ss->tuned.find(key);
     if (it != ss->tuned.end()) return it->second;
     // Time every candidate on this call's own data (all of them write the
@@ -364,7 +256,7 @@ int pick_config(mdiscr_ctx* ctx, const uint32_t* d_in, uint32_t* d_out, uint64_t
     const bool saved_timing = ctx->timing;
     ctx->timing = false;
     for (int c = 0; c < kNumConfigs; ++c) {
-        if (!config_available(mask, nbits, c)) continue;
+        if (!config_available(c)) continue;
         float mn = 1e30f;
         for (int rep = 0; rep < 4; ++rep) {
             mdiscr_status st = ctx_hip(ctx, hipEventRecord(a, s), "hipEventRecord");
@@ -654,10 +546,10 @@ mdiscr_status mdiscr_partition_batch(mdiscr_ctx* ctx, const mdiscr_request* reqs
     if (st) return st;
     const uint32_t nbits = class_bits(mmax);
     int cfg = 0;
-    auto it = ss->tuned.find(bucket_key(all_pow2, nbits, total, nreq));
-    if (const int c = forced_config(); c >= 0 && config_available(all_pow2, nbits, c))
+    auto it = ss->tuned.find(bucket_key(all_pow2, mmax, total, nreq));
+    if (const int c = forced_config(); c >= 0 && config_available(c))
         cfg = c;
-    else if (it != ss->tuned.end() && config_available(all_pow2, nbits, it->second))
+    else if (it != ss->tuned.end() && config_available(it->second))
         cfg = it->second;
     SegArgs a;
     a.staged = static_cast<const SegDesc*>(ss->host_desc_dev);  // the planner copies them
@@ -680,6 +572,7 @@ mdiscr_status mdiscr_partition_batch(mdiscr_ctx* ctx, const mdiscr_request* reqs
         uint64_t est = 0;
         for (uint32_t i = 0; i < nreq; ++i) est += reqs[i].n <= wt ? 1 : (reqs[i].n + tile - 1) / tile;
         a.grid_cap = grid_cap(est);
+        a.item_cap = grid_cap(2 * (uint64_t)nreq + total / kMinTile + 2);
         a.num_cu = (uint32_t)ctx->num_cu;
     }
     TimedLaunch tl;

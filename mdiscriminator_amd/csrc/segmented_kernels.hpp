@@ -82,10 +82,16 @@ struct OffSrc {
     uint32_t* counts;      // m per segment, or null
     uint32_t m;
     mdiscr_divmagic g;
+    uint64_t n;            // elements of in / out
 };
 
+// Segment s = [off[s], off[s+1]), clamped to a range inside [0, n): offsets
+// that decrease or run past n (unchecked when the caller gave no host copy)
+// give segments that are empty or cut at n, never an access outside the
+// arrays.
 __device__ __forceinline__ SegDesc desc_from_offsets(const OffSrc& os, uint32_t s) {
-    const uint64_t a = os.off[s], b = os.off[s + 1];
+    const uint64_t a = min(os.off[s], os.n);
+    const uint64_t b = min(max(os.off[s + 1], a), os.n);
     SegDesc d;
     d.in = os.in + a;
     d.out = os.out + a;
@@ -135,7 +141,8 @@ __global__ __launch_bounds__(THREADS) void k_seg_plan(const SegDesc* __restrict_
                                                       SegLong* __restrict__ longs,
                                                       SegPlan* __restrict__ plan,
                                                       uint32_t* __restrict__ counts, uint32_t ld,
-                                                      uint32_t mrows, uint32_t expand_here) {
+                                                      uint32_t mrows, uint32_t expand_here,
+                                                      uint32_t cap_items, uint32_t cap_rows) {
     // Segments are grouped by index, NW per group; a group's short segments
     // (len <= 64*R) form one W item whose bit w marks segment first+w.  Each
     // thread owns SPT consecutive segments (whole groups) per pass.  A
@@ -178,7 +185,9 @@ __global__ __launch_bounds__(THREADS) void k_seg_plan(const SegDesc* __restrict_
                     desc[base + i] = d;
                     slen[i + i / SPT] = d.len;
                 } else {
-                    slen[i + i / SPT] = (uint32_t)(os.off[base + i + 1] - os.off[base + i]);
+                    const uint64_t a = min(os.off[base + i], os.n);
+                    const uint64_t b = min(max(os.off[base + i + 1], a), os.n);
+                    slen[i + i / SPT] = (uint32_t)(b - a);
                 }
             }
         } else {
@@ -223,14 +232,17 @@ __global__ __launch_bounds__(THREADS) void k_seg_plan(const SegDesc* __restrict_
 #pragma unroll
             for (uint32_t w = 0; w < NW; ++w)
                 if (s0 + g + w < nseg && len[g + w] <= WT) mask |= 1u << w;
-            if (mask) items[ib++] = SegItem{s0 + g, mask, 0u, 0u};
+            if (mask) {
+                if (ib < cap_items) items[ib] = SegItem{s0 + g, mask, 0u, 0u};
+                ++ib;
+            }
 #pragma unroll
             for (uint32_t w = 0; w < NW; ++w) {
                 const uint32_t k = g + w;
                 if (s0 + k >= nseg || len[k] <= WT) continue;
                 const uint32_t tiles = (len[k] + TILE - 1) / TILE;
                 if (tiles == 1) {
-                    items[ib] = SegItem{s0 + k, 0u, 1u, 0u};
+                    if (ib < cap_items) items[ib] = SegItem{s0 + k, 0u, 1u, 0u};
                 } else {
                     const SegLong L{s0 + k, ib, tiles, rb};
                     if (expand_here) slong[lb++] = L;
@@ -244,13 +256,20 @@ __global__ __launch_bounds__(THREADS) void k_seg_plan(const SegDesc* __restrict_
         row_carry += rtot;
         long_carry += ltot;
     }
-    if (threadIdx.x == 0) *plan = SegPlan{item_carry, row_carry, long_carry};
+    // Segments whose lengths add up to more than the arrays hold (offsets that
+    // overlap; unchecked on the host without a host copy) would need more
+    // items or rows than the workspace has: such a plan is left empty, and no
+    // later kernel of the call touches memory.
+    const bool over = item_carry > cap_items || row_carry > cap_rows;
+    if (threadIdx.x == 0)
+        *plan = over ? SegPlan{0u, 0u, 0u} : SegPlan{item_carry, row_carry, long_carry};
     // A zero after the last row: the scan's exclusive prefix there is the total.
-    for (uint32_t c = threadIdx.x; c < mrows; c += THREADS) counts[(size_t)c * ld + row_carry] = 0u;
+    if (!over)
+        for (uint32_t c = threadIdx.x; c < mrows; c += THREADS) counts[(size_t)c * ld + row_carry] = 0u;
     // Few segments (a serving batch): write the multi-tile segments' T items
     // here, the whole block on one segment at a time, instead of launching
     // k_seg_expand.
-    if (expand_here) {
+    if (expand_here && !over) {
         __syncthreads();
         for (uint32_t k = 0; k < long_carry; ++k) {
             const SegLong L = slong[k];

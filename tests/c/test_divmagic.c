@@ -1,5 +1,6 @@
-/* Checks mdiscr_divmagic_mod(x) == x % d for d in [1,64] and some large d on
- * edge values and 2^24 pseudo-random x per divisor. */
+/* Checks mdiscr_divmagic_mod(x) == x % d for every d in [1,64] and some
+ * larger d, per divisor on 12 edge values (0, d-1, d, 2^31, 2^32-1, the last
+ * multiple of d, ...), on every x below 2^20, and on 2^22 pseudo-random x. */
 #include <stdio.h>
 #include <stdint.h>
 #include "divmagic.h"

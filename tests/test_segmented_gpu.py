@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 KAT = json.load(open(os.path.join(HERE, "golden", "kat.json")))
-NUM_CONFIGS = 8
+NUM_CONFIGS = 24
 
 
 def run_offsets(ctx, x, off, discr, stream=None, counts=True):
@@ -78,20 +78,54 @@ def test_segmented_lognormal_mix(ctx, m):
 
 @pytest.mark.parametrize("cfg", list(range(NUM_CONFIGS)))
 def test_every_tuned_config(ctx, cfg):
-    """Every candidate configuration of the autotuner gives the exact result."""
+    """Every candidate configuration of the autotuner gives the exact result,
+    for power-of-two (mask) and other (magic-divide mod) class counts up to 64."""
     import mdiscriminator_amd as md
     os.environ["MDISCR_SEG_CONFIG"] = str(cfg)
     try:
-        for m in (2, 4, 8, 16):
+        for discr in (md.Mask(4), md.Mod(5), md.Mask(16), md.Mod(33), md.Mask(64)):
+            m = discr.m
             L = lengths_mix(cfg * 31 + m, 300, 2000)
             L[5] = 70000    # a many-tile segment
             L[6] = 1        # and neighbours of every size
             L[7] = 2047
+            L[8] = 40000    # a tile-multiple of every candidate (1024 * 2^k)
+            L[9] = 32768
             off = W.offsets_from_lengths(L)
             x = W.uniform_u31(int(off[-1]), cfg + m)
-            check_offsets(ctx, x, off, md.Mask(m), f"cfg={cfg} m={m}")
+            print(f"segmented cfg={cfg} m={m} n={off[-1]}", flush=True)
+            check_offsets(ctx, x, off, discr, f"cfg={cfg} m={m}")
     finally:
         del os.environ["MDISCR_SEG_CONFIG"]
+
+
+def test_autotuner_buckets_and_cache(ctx, tmp_path):
+    """A call of a new (kind, m, size) bucket times every candidate on its own
+    data, keeps the fastest and records it, with its name, in
+    MDISCR_TUNE_CACHE; the result is exact whichever candidate won."""
+    import mdiscriminator_amd as md
+    cache = tmp_path / "tune.txt"
+    os.environ["MDISCR_TUNE_CACHE"] = str(cache)
+    try:
+        c2 = md.Context(0)   # a fresh tuning table
+        try:
+            for discr in (md.Mod(33), md.Mask(8)):
+                L = lengths_mix(400 + discr.m, 500, 3000)
+                off = W.offsets_from_lengths(L)
+                x = W.uniform_u31(int(off[-1]), 7 + discr.m)
+                print(f"autotune m={discr.m} n={off[-1]}", flush=True)
+                check_offsets(c2, x, off, discr, f"tuned m={discr.m}")
+        finally:
+            c2.close()
+    finally:
+        del os.environ["MDISCR_TUNE_CACHE"]
+    lines = cache.read_text().splitlines()
+    keys = {ln.split("=")[0] for ln in lines}
+    assert any(k.startswith("u32 mod m33 ") for k in keys), lines
+    assert any(k.startswith("u32 mask m8 ") for k in keys), lines
+    for ln in lines:
+        idx, name = ln.split("=")[1].split()
+        assert 0 <= int(idx) < NUM_CONFIGS and name.startswith("b")
 
 
 def test_segment_shapes(ctx):
@@ -213,6 +247,40 @@ def test_segmented_invalid_offsets(ctx):
     with pytest.raises(md.MdiscrError):
         ctx.partition_segmented(d_in, d_out, torch.from_numpy(bad.astype(np.int64)).to("cuda:0"),
                                 md.Mod4, h_seg_off=bad)
+
+
+def test_segmented_unchecked_bad_offsets_stay_in_bounds(ctx):
+    """Device offsets that decrease, run past n or overlap, with no host copy
+    to check them: each segment is clamped into [0, n), and a plan whose
+    segments add up to more than the workspace holds is left empty.  The
+    call returns, and nothing outside d_out is written (sentinels around it)."""
+    import torch
+    import mdiscriminator_amd as md
+    n = 1 << 20
+    x = W.uniform_u31(n, 77)
+    cases = {
+        "decreasing": np.array([0, 500000, 400000, n], dtype=np.uint64),
+        "past n": np.array([0, 300000, n + 5000, 3 * n], dtype=np.uint64),
+        "overlapping": np.array([0, n, 0, n], dtype=np.uint64),
+        "overlapping beyond the queue": np.array([0, n] * 6 + [0], dtype=np.uint64),
+    }
+    for name, off in cases.items():
+        print(f"unchecked offsets: {name}", flush=True)
+        guard = 4096
+        d_all = torch.full((n + 2 * guard,), -7, dtype=torch.int32, device="cuda:0")
+        d_out = d_all[guard:guard + n]
+        d_in = to_dev(x)
+        d_off = torch.from_numpy(off.astype(np.int64)).to("cuda:0")
+        nseg = len(off) - 1
+        d_cnt = torch.full((nseg * 4 + 2 * guard,), -7, dtype=torch.int32, device="cuda:0")
+        ctx.partition_segmented(d_in, d_out, d_off, md.Mod4,
+                                d_seg_counts=d_cnt[guard:guard + nseg * 4])
+        torch.cuda.synchronize()
+        assert torch.all(d_all[:guard] == -7) and torch.all(d_all[guard + n:] == -7), name
+        assert torch.all(d_cnt[:guard] == -7) and torch.all(d_cnt[guard + nseg * 4:] == -7), name
+    # the context works normally afterwards
+    off = W.offsets_from_lengths(lengths_mix(5, 200, 3000))
+    check_offsets(ctx, W.uniform_u31(int(off[-1]), 6), off, md.Mod4, "after bad offsets")
 
 
 def test_full_size_10k_lognormal(ctx):

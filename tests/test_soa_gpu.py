@@ -149,7 +149,7 @@ def test_array_elements(ctx, length):
         x = W.uniform_u31(n * length, 900 + length + n)
         d_in = torch.from_numpy(x.view(np.int32)).to("cuda:0")
         d_out = torch.full_like(d_in, -1)
-        d_cnt = torch.empty(m, dtype=torch.int32, device="cuda:0")
+        d_cnt = torch.full((m,), -1, dtype=torch.int32, device="cuda:0")
         print(f"arrays len={length} n={n} m={m}", flush=True)
         ctx.partition_arrays(d_in, d_out, length, md.Mod(m), d_counts=d_cnt)
         torch.cuda.synchronize()
@@ -207,7 +207,7 @@ def test_array_elements_full_size(ctx):
     d_in = W.uniform_u31_torch(n * length, 77, "cuda:0")
     x = d_in.cpu().numpy().view(np.uint32)
     d_out = torch.empty_like(d_in)
-    d_cnt = torch.empty(m, dtype=torch.int32, device="cuda:0")
+    d_cnt = torch.full((m,), -1, dtype=torch.int32, device="cuda:0")
     print("arrays 2^26 x 4", flush=True)
     ctx.partition_arrays(d_in, d_out, length, md.Mask(m), d_counts=d_cnt)
     torch.cuda.synchronize()
@@ -283,7 +283,7 @@ def test_soa_maximum_size(ctx):
         d_key[s:e] = i.to(torch.int32)
         d_pay[s:e] = ((i >> 3) & 0xFF).to(torch.uint8)
     o_key, o_pay = torch.empty_like(d_key), torch.empty_like(d_pay)
-    d_cnt = torch.empty(m, dtype=torch.int32, device="cuda:0")
+    d_cnt = torch.full((m,), -1, dtype=torch.int32, device="cuda:0")
     print("soa max size: partition", flush=True)
     ctx.partition_soa([d_key, d_pay], [o_key, o_pay], md.Mask(m), d_counts=d_cnt)
     torch.cuda.synchronize()
@@ -316,7 +316,7 @@ def test_array_elements_word_aligned_only(ctx, length):
         d_in = big_in[1:1 + n * length]
         big_out = torch.full((n * length + 4,), -1, dtype=torch.int32, device="cuda:0")
         d_out = big_out[1:1 + n * length]
-        d_cnt = torch.empty(m, dtype=torch.int32, device="cuda:0")
+        d_cnt = torch.full((m,), -1, dtype=torch.int32, device="cuda:0")
         print(f"arrays word-aligned len={length} n={n} m={m}", flush=True)
         ctx.partition_arrays(d_in, d_out, length, md.Mod(m), d_counts=d_cnt)
         torch.cuda.synchronize()
