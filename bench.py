@@ -110,6 +110,36 @@ def cpu_baseline(quick=False):
     return head
 
 
+def first_steps(ctx, step, stream, steps):
+    """The first `steps` steps, each preceded by a shader-clock probe."""
+    import ctypes
+    import torch
+    probe = ctypes.CDLL(os.path.join(ROOT, "mdiscriminator_amd", "libclkprobe.so"))
+    khz = ctypes.c_int()
+    if probe.mdiscr_wall_clock_khz(ctypes.byref(khz)) != 0 or khz.value <= 0:
+        raise RuntimeError("wall clock rate unavailable")
+    buf = torch.zeros(2 * steps, dtype=torch.int64, device="cuda:0")
+    span = khz.value * 20 // 1000  # 20 us of wall clock
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    for i in range(steps):
+        if probe.mdiscr_clk_probe(ctypes.c_void_p(buf.data_ptr()), i, span,
+                                  ctypes.c_void_p(stream.cuda_stream)) != 0:
+            raise RuntimeError("clock probe launch failed")
+        ev[i].record()
+        step()
+    ev[steps].record()
+    torch.cuda.synchronize()
+    b = buf.cpu().tolist()
+    rows = []
+    for i in range(steps):
+        ticks, wall = b[2 * i], b[2 * i + 1]
+        rows.append({"step": i, "sclk_mhz": round(ticks / (wall / (khz.value * 1e3)) / 1e6, 1),
+                     "ms": round(ev[i].elapsed_time(ev[i + 1]), 4)})
+        log(f"step {i:3d}: shader clock {rows[-1]['sclk_mhz']:7.1f} MHz, "
+            f"step {rows[-1]['ms']:.4f} ms")
+    print(json.dumps({"first_steps": rows}), flush=True)
+
+
 def run_driver(args, timeout):
     """Runs a C host driver of the C ABI and returns its JSON line (or an error)."""
     import subprocess
@@ -144,6 +174,13 @@ def main():
                     help="for rocprofv3 counter runs: a few partition steps, then an "
                          "elementwise copy of the same 2^28 uint32 (known 1 GiB read + "
                          "1 GiB write) that calibrates FETCH_SIZE / WRITE_SIZE")
+    ap.add_argument("--first-steps", type=int, default=0,
+                    help="the first N plain steps after the inputs are generated, each "
+                         "preceded by a one-lane shader-clock probe (about 20 us): prints "
+                         "the clock and the event-timed step per step, then exits (under "
+                         "rocprofv3 --kernel-trace, the trace gives each kernel's duration)")
+    ap.add_argument("--profile-m", type=int, default=M,
+                    help="with --profile-mode: the class count of the plain steps (x & (m-1))")
     ap.add_argument("--profile-records", action="store_true",
                     help="with --profile-mode: also a few SoA partitions of 2^26 "
                          "(key, f32, index) records at m = 16, 32, 64")
@@ -174,9 +211,15 @@ def main():
     def step_zipf():
         ctx.partition_u32(d_zipf, d_out, md.Mod8, d_counts=d_cnt, stream=stream)
 
+    if args.first_steps:
+        first_steps(ctx, step, stream, args.first_steps)
+        ctx.close()
+        return
+
     if args.profile_mode:
+        pm = md.Mask(args.profile_m)
         for _ in range(args.steps):
-            step()
+            ctx.partition_u32(d_in, d_out, pm, stream=stream)
         torch.cuda.synchronize()
         if args.profile_records:
             ns = 1 << 26

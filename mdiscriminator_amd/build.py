@@ -78,6 +78,20 @@ def build_library(verbose=False):
     return LIB
 
 
+PROBE_LIB = os.path.join(PKG, "libclkprobe.so")
+
+
+def build_probe(verbose=False):
+    """The shader-clock probe bench.py --first-steps samples between steps."""
+    cmd = [_hipcc()] + HIPCC_FLAGS + ["-o", PROBE_LIB, os.path.join(CSRC, "clock_probe.hip")]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("building libclkprobe.so failed")
+
+
 HOST_PROGRAMS = {"mdiscr_segbench": "mdiscr_segbench.c", "mdiscr_servebench": "mdiscr_servebench.c"}
 BIN = os.path.join(PKG, "bin")
 
@@ -130,6 +144,7 @@ def build_oracle(verbose=False):
 
 def build_all(verbose=False):
     build_library(verbose)
+    build_probe(verbose)
     build_host_programs(verbose)
     build_oracle(verbose)
     build_test_programs(verbose)

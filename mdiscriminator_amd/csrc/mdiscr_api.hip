@@ -264,9 +264,6 @@ namespace {
 #ifndef MDISCR_PLAIN_R
 #define MDISCR_PLAIN_R 16
 #endif
-#ifndef MDISCR_PLAIN_CT
-#define MDISCR_PLAIN_CT 1
-#endif
 // Tiles per count workgroup.  One tile each reads fastest (2^28 keys, m=8,
 // interleaved A/B: count 0.1515 / 0.1523 / 0.1572 / 0.1629 ms for 1 / 2 / 4 /
 // 8 tiles; whole call 0.538 vs 0.547 ms for 1 vs 8).
@@ -282,13 +279,13 @@ namespace {
 #ifndef MDISCR_PLAIN_MINW
 #define MDISCR_PLAIN_MINW 1
 #endif
+// The ballot-ranked path's tile up to 16 classes (an input that is not
+// 16-byte aligned): kNW waves x kR rounds, one tile per partition workgroup.
 constexpr int kNW = MDISCR_PLAIN_NW;  // waves per workgroup
 constexpr int kR = MDISCR_PLAIN_R;    // rounds (elements per lane per tile)
-constexpr int kCT = MDISCR_PLAIN_CT;  // tiles per partition workgroup
 constexpr int kCCT = MDISCR_COUNT_CT; // tiles per count workgroup
 constexpr bool kPrefetch = MDISCR_PLAIN_PREFETCH != 0;
-using PlainGeom = Geom<kNW, kR, kCT>;
-constexpr uint32_t kTile = PlainGeom::kTile;
+constexpr uint32_t kTile = Geom<kNW, kR, 1>::kTile;  // the smallest plain tile
 
 // The quad path (16-byte loads, packed-counter ranks) serves up to 16
 // classes on a 16-byte aligned input, in tiles of its own size; the
@@ -400,17 +397,41 @@ mdiscr_status launch_plain_quad(mdiscr_ctx* ctx, const uint32_t* in, uint32_t* o
     return ctx_hip(ctx, hipGetLastError(), "plain partition launch");
 }
 
-template <int NBITS, class Discr>
-mdiscr_status launch_plain(mdiscr_ctx* ctx, const uint32_t* in, uint32_t* out, uint32_t n,
-                           Discr dis, uint32_t m, uint32_t* user_counts, uint64_t* offsets,
-                           hipStream_t s, uint32_t* ws) {
-    if constexpr (NBITS <= 4 && MDISCR_PLAIN_QUAD != 0) {
-        if (((uintptr_t)in & 15u) == 0)
-            return launch_plain_quad<NBITS>(ctx, in, out, n, dis, m, user_counts, offsets, s, ws);
-    }
-    const uint32_t T = (uint32_t)(((uint64_t)n + kTile - 1) / kTile);
+// Geometry of the ballot-ranked path from MDISCR_BIG_MIN_M classes: NW waves
+// x R rounds per tile, CNW count waves per tile, WGCU partition workgroups
+// per CU (0: as many as fit).  With many classes a 4096-key tile's class runs
+// are short (64 keys at m = 64); 8192-key tiles of 8 waves, two per CU, write
+// longer runs.  2^28 keys, interleaved A/B, partition ms for 4096-key tiles
+// (4 waves, no cap) against this geometry: m = 64 0.638 / 0.531, m = 33
+// 0.536 / 0.577, m = 32 0.483 / 0.484.
+#ifndef MDISCR_BIG_MIN_M
+#define MDISCR_BIG_MIN_M 48
+#endif
+#ifndef MDISCR_BIG_NW
+#define MDISCR_BIG_NW 8
+#endif
+#ifndef MDISCR_BIG_R
+#define MDISCR_BIG_R 16
+#endif
+#ifndef MDISCR_BIG_COUNT_NW
+#define MDISCR_BIG_COUNT_NW 8
+#endif
+#ifndef MDISCR_BIG_WG_PER_CU
+#define MDISCR_BIG_WG_PER_CU 2
+#endif
+
+// The ballot-ranked path in tiles of NW * 64 * R elements (count in the same
+// tiles with CNW waves), or the partition alone for an input of one tile.
+template <int NW, int R, int CNW, int WGCU, int NBITS, class Discr>
+mdiscr_status launch_plain_ballot(mdiscr_ctx* ctx, const uint32_t* in, uint32_t* out, uint32_t n,
+                                  Discr dis, uint32_t m, uint32_t* user_counts,
+                                  uint64_t* offsets, hipStream_t s, uint32_t* ws) {
+    constexpr uint32_t TILE = (uint32_t)NW * kWave * R;
+    constexpr int CR = (int)(TILE / (CNW * kWave));
+    static_assert(CR * CNW * kWave == (int)TILE, "count and partition tiles agree");
+    static_assert(TILE >= kTile, "the workspace is sized by the smallest tile");
+    const uint32_t T = (uint32_t)(((uint64_t)n + TILE - 1) / TILE);
     const uint32_t ld = (T + 3u) & ~3u;     // 16-byte aligned class rows
-    const uint32_t Kp = (T + kCT - 1) / kCT;
     uint32_t* counts = ws;                      // m * ld
     uint32_t* prefix = ws + (size_t)m * ld;     // m * ld
     uint32_t* totals = ws + (size_t)2 * m * ld; // m
@@ -419,21 +440,41 @@ mdiscr_status launch_plain(mdiscr_ctx* ctx, const uint32_t* in, uint32_t* out, u
         // One tile: the partition counts it itself (3 launches -> 1; a call on
         // one tile or less is bound by launch latency, not bandwidth).
         timing_begin(ctx, s, tl);
-        hipLaunchKernelGGL((k_partition<kNW, kR, kCT, NBITS, kPrefetch, MDISCR_PLAIN_WB,
+        hipLaunchKernelGGL((k_partition<NW, R, 1, NBITS, kPrefetch, MDISCR_PLAIN_WB,
                                         MDISCR_PLAIN_MINW, Discr, true>),
-                           dim3(1), dim3(kNW * kWave), 0, s, in, out, n, dis, nullptr, nullptr,
+                           dim3(1), dim3(NW * kWave), 0, s, in, out, n, dis, nullptr, nullptr,
                            0u, m, user_counts, offsets);
         timing_end(ctx, s, tl, "partition");
         return ctx_hip(ctx, hipGetLastError(), "plain partition launch");
     }
-    launch_count_scan<kNW, kR, NBITS>(ctx, in, n, dis, m, user_counts, s, counts, prefix, totals,
+    launch_count_scan<CNW, CR, NBITS>(ctx, in, n, dis, m, user_counts, s, counts, prefix, totals,
                                       T, ld);
+    constexpr auto kern = k_partition<NW, R, 1, NBITS, kPrefetch, MDISCR_PLAIN_WB,
+                                      MDISCR_PLAIN_MINW, Discr>;
+    static const size_t pad = lds_pad_for(kern, WGCU);
     timing_begin(ctx, s, tl);
-    hipLaunchKernelGGL((k_partition<kNW, kR, kCT, NBITS, kPrefetch, MDISCR_PLAIN_WB, MDISCR_PLAIN_MINW, Discr>),
-                       dim3(Kp), dim3(kNW * kWave), 0, s, in, out, n, dis, prefix, totals, ld, m,
-                       nullptr, offsets);
+    hipLaunchKernelGGL(kern, dim3(T), dim3(NW * kWave), pad, s, in, out, n, dis, prefix, totals,
+                       ld, m, nullptr, offsets);
     timing_end(ctx, s, tl, "partition");
     return ctx_hip(ctx, hipGetLastError(), "plain partition launch");
+}
+
+template <int NBITS, class Discr>
+mdiscr_status launch_plain(mdiscr_ctx* ctx, const uint32_t* in, uint32_t* out, uint32_t n,
+                           Discr dis, uint32_t m, uint32_t* user_counts, uint64_t* offsets,
+                           hipStream_t s, uint32_t* ws) {
+    if constexpr (NBITS <= 4 && MDISCR_PLAIN_QUAD != 0) {
+        if (((uintptr_t)in & 15u) == 0)
+            return launch_plain_quad<NBITS>(ctx, in, out, n, dis, m, user_counts, offsets, s, ws);
+    }
+    if constexpr (NBITS == 6) {
+        if (m >= MDISCR_BIG_MIN_M)
+            return launch_plain_ballot<MDISCR_BIG_NW, MDISCR_BIG_R, MDISCR_BIG_COUNT_NW,
+                                       MDISCR_BIG_WG_PER_CU, NBITS>(ctx, in, out, n, dis, m,
+                                                                    user_counts, offsets, s, ws);
+    }
+    return launch_plain_ballot<kNW, kR, kNW, 0, NBITS>(ctx, in, out, n, dis, m, user_counts,
+                                                       offsets, s, ws);
 }
 
 template <class Discr>

@@ -6,10 +6,12 @@
  * nreq partition requests (default 50000) with sizes log-uniform over
  * [1Ki, 16Mi] uint32 values (uniform in [0, 2^31)), m uniform in {2,4,8,16},
  * class = x & (m-1).  Inputs are views into one resident 64Mi-value pool.
- * Two capacities are measured on all requests submitted at once: the plain
- * per-request path (one mdiscr_partition_u32 call + stream sync each) and the
- * continuous-batching server (mdiscr_server_*).  Poisson arrival schedules
- * at `load` (default 0.8) times each capacity are then served by both, and
+ * Three capacities are measured on all requests submitted at once: the plain
+ * per-request path (one mdiscr_partition_u32 call + stream sync each), the
+ * plain path on nstreams streams (one context each, requests dealt
+ * round-robin, no host wait per request) and the continuous-batching server
+ * (mdiscr_server_*).  Poisson arrival schedules at `load` (default 0.8)
+ * times the plain and the server capacity are then served by all three, and
  * one JSON line reports p50/p99 latency (arrival -> result ready) and the
  * aggregate Gelem/s of each, plus spot checks that served outputs equal the
  * plain path's bit for bit.
@@ -192,6 +194,66 @@ static Result run_plain(Req* r, int nreq, const uint32_t* pool, mdiscr_ctx* ctx,
     return res;
 }
 
+/* The plain path on nstreams streams: one context, stream and output buffer
+ * per stream, requests dealt round-robin as they arrive, no host wait per
+ * request.  A second thread records each request's completion by polling
+ * the event recorded after it. */
+typedef struct {
+    hipEvent_t* ev;
+    volatile int submitted;
+    int nreq;
+    double* done_t;
+} Poller;
+
+static void* poll_events(void* arg) {
+    Poller* p = (Poller*)arg;
+    int next = 0;
+    while (next < p->nreq) {
+        const int sub = __atomic_load_n(&p->submitted, __ATOMIC_ACQUIRE);
+        for (int j = next; j < sub; j++) {
+            if (p->done_t[j] > 0) continue;
+            if (hipEventQuery(p->ev[j]) == hipSuccess) p->done_t[j] = now_s();
+        }
+        while (next < sub && p->done_t[next] > 0) next++;
+        if (next < p->nreq) sleep_until(now_s() + 5e-6);
+    }
+    return NULL;
+}
+
+static Result run_plain_streams(Req* r, int nreq, const uint32_t* pool, int k, mdiscr_ctx** ctxs,
+                                hipStream_t* sts, uint32_t** outs, hipEvent_t* ev) {
+    Result res = {0};
+    double* done_t = (double*)calloc(nreq, sizeof(double));
+    Poller pl = {ev, 0, nreq, done_t};
+    pthread_t th;
+    pthread_create(&th, NULL, poll_events, &pl);
+    const double t0 = now_s() + 0.02;
+    for (int i = 0; i < nreq; i++) {
+        sleep_until(t0 + r[i].t_arrive);
+        const int s = i % k;
+        const mdiscr_discr d = {MDISCR_DISCR_MASK, r[i].m};
+        if (mdiscr_partition_u32(ctxs[s], pool + r[i].off, outs[s], r[i].n, d, NULL, sts[s]) !=
+                MDISCR_OK ||
+            hipEventRecord(ev[i], sts[s]) != hipSuccess)
+            exit(1);
+        __atomic_store_n(&pl.submitted, i + 1, __ATOMIC_RELEASE);
+    }
+    pthread_join(th, NULL);
+    double* lat = (double*)malloc(nreq * sizeof(double));
+    double t_last = t0;
+    for (int i = 0; i < nreq; i++) {
+        lat[i] = done_t[i] - (t0 + r[i].t_arrive);
+        if (done_t[i] > t_last) t_last = done_t[i];
+    }
+    res.p50_ms = pct(lat, nreq, 0.5) * 1e3;
+    res.p99_ms = pct(lat, nreq, 0.99) * 1e3;
+    res.gelem_s = g_total / (t_last - t0) / 1e9;
+    res.identical = 1;
+    free(lat);
+    free(done_t);
+    return res;
+}
+
 static void schedule(Req* r, int nreq, double lam, uint64_t seed) {
     double t = 0;
     for (int i = 0; i < nreq; i++) {
@@ -239,19 +301,35 @@ int main(int argc, char** argv) {
     mdiscr_server_config cfg = {0, (uint64_t)(budget_gib * (1ull << 30)), (uint32_t)nstreams,
                                 1024, 1ull << 28, 0, (uint64_t)(reserve_gib * (1ull << 30))};
 
+    /* the plain path on nstreams streams (one context each, no host wait) */
+    mdiscr_ctx** pctx = (mdiscr_ctx**)calloc(nstreams, sizeof(mdiscr_ctx*));
+    hipStream_t* psts = (hipStream_t*)calloc(nstreams, sizeof(hipStream_t));
+    uint32_t** pouts = (uint32_t**)calloc(nstreams, sizeof(uint32_t*));
+    for (int k = 0; k < nstreams; k++)
+        if (mdiscr_ctx_create(&pctx[k], 0) != MDISCR_OK ||
+            hipStreamCreateWithFlags(&psts[k], hipStreamNonBlocking) != hipSuccess ||
+            hipMalloc((void**)&pouts[k], (16ull << 20) * 4) != hipSuccess)
+            return 1;
+    hipEvent_t* pev = (hipEvent_t*)calloc(nreq, sizeof(hipEvent_t));
+    for (int i = 0; i < nreq; i++)
+        if (hipEventCreateWithFlags(&pev[i], hipEventDisableTiming) != hipSuccess) return 1;
+
     /* capacities: everything submitted at t = 0 */
     schedule(r, nreq, 0, 0);
     (void)run_plain(r, nreq < 2000 ? nreq : 2000, d_pool, ctx, st, d_plain);  /* warm-up */
+    (void)run_plain_streams(r, nreq < 2000 ? nreq : 2000, d_pool, nstreams, pctx, psts, pouts, pev);
     const Result cap_plain = run_plain(r, nreq, d_pool, ctx, st, d_plain);
+    const Result cap_streams = run_plain_streams(r, nreq, d_pool, nstreams, pctx, psts, pouts, pev);
     const Result cap_srv = run_server(r, nreq, d_pool, &cfg, ctx, st, d_plain);
     const double mean_n = (double)g_total / nreq;
-    fprintf(stderr, "capacity: plain %.1f Gelem/s, server %.1f Gelem/s\n", cap_plain.gelem_s,
-            cap_srv.gelem_s);
+    fprintf(stderr, "capacity: plain %.1f, plain on %d streams %.1f, server %.1f Gelem/s\n",
+            cap_plain.gelem_s, nstreams, cap_streams.gelem_s, cap_srv.gelem_s);
 
     printf("{\"nreq\": %d, \"total_elems\": %llu, \"load\": %.2f, \"budget_gib\": %.1f, "
-           "\"nstreams\": %d, \"capacity_gelem_s\": {\"plain\": %.2f, \"server\": %.2f}, ",
+           "\"nstreams\": %d, \"capacity_gelem_s\": {\"plain\": %.2f, \"plain_streams\": %.2f, "
+           "\"server\": %.2f}, ",
            nreq, (unsigned long long)g_total, load, budget_gib, nstreams, cap_plain.gelem_s,
-           cap_srv.gelem_s);
+           cap_streams.gelem_s, cap_srv.gelem_s);
     int identical = cap_srv.identical;
     const char* names[2] = {"at_plain_capacity", "at_server_capacity"};
     const double caps[2] = {cap_plain.gelem_s, cap_srv.gelem_s};
@@ -260,14 +338,22 @@ int main(int argc, char** argv) {
         schedule(r, nreq, lam, seed + 2 + k);
         const Result s = run_server(r, nreq, d_pool, &cfg, ctx, st, d_plain);
         const Result p = run_plain(r, nreq, d_pool, ctx, st, d_plain);
+        const Result q = run_plain_streams(r, nreq, d_pool, nstreams, pctx, psts, pouts, pev);
         identical &= s.identical;
         printf("\"%s\": {\"arrival_rate_req_s\": %.1f, \"offered_gelem_s\": %.2f, ", names[k], lam,
                load * caps[k]);
         print_result("server", s, 0);
-        print_result("plain", p, 1);
+        print_result("plain", p, 0);
+        print_result("plain_streams", q, 1);
         printf("}, ");
     }
     printf("\"identical_spot_checks\": %s}\n", identical ? "true" : "false");
+    for (int i = 0; i < nreq; i++) hipEventDestroy(pev[i]);
+    for (int k = 0; k < nstreams; k++) {
+        mdiscr_ctx_destroy(pctx[k]);
+        hipStreamDestroy(psts[k]);
+        hipFree(pouts[k]);
+    }
     mdiscr_ctx_destroy(ctx);
     return identical ? 0 : 3;
 }

@@ -33,8 +33,11 @@ def check(ctx, x, discr, label=""):
     assert i < 0, f"{label} first mismatch at {i}: {out[i]} vs {ref_out[i]}"
 
 
-EDGE_N = [0, 1, 2, 63, 64, 65, 1023, 1024, 1025, TILE - 1, TILE, TILE + 1, 3 * TILE + 17,
-          100003]
+# TILE: the ballot path's tile; QTILE: the 16-byte-load path's (up to 16
+# classes) and the ballot path's from 48 classes on.
+QTILE = 8192
+EDGE_N = [0, 1, 2, 63, 64, 65, 1023, 1024, 1025, TILE - 1, TILE, TILE + 1, QTILE - 1, QTILE,
+          QTILE + 1, 3 * TILE + 17, 2 * QTILE + 3, 100003]
 
 
 @pytest.mark.parametrize("m", [1, 2, 3, 4, 5, 7, 8, 13, 16, 31, 32, 33, 63, 64])
@@ -115,7 +118,7 @@ def test_reference_shaped_call(ctx):
     assert np.array_equal(from_dev(d_out2), ref)
 
 
-@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 1000, 4095, 4096, 4097])
+@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 1000, 4095, 4096, 4097, 8191, 8192, 8193])
 def test_one_tile_inputs(ctx, n):
     """Inputs of at most one tile take a single launch that counts its own
     tile; the counts come back through both the synchronous reference-shaped
