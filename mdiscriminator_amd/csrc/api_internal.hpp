@@ -64,6 +64,25 @@ uint32_t class_bits(uint32_t m);
 // unknown op.
 bool pred_window(mdiscr_pred p, uint32_t w[4]);
 
+// Dynamic LDS bytes that make `kernel` (static LDS as compiled) fit at most
+// `per_cu` workgroups per CU: the CU's LDS divided by per_cu, less the static part.
+template <typename K>
+inline size_t lds_pad_for(K kernel, int per_cu) {
+    if (per_cu <= 0) return 0;
+    hipFuncAttributes fa{};
+    int dev = 0, lds_cu = 0;
+    if (hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(kernel)) != hipSuccess ||
+        hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) !=
+            hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    const size_t target = (size_t)lds_cu / (size_t)per_cu;
+    const size_t pad = target > fa.sharedSizeBytes ? target - fa.sharedSizeBytes : 0;
+    return pad & ~(size_t)255;
+}
+
 inline bool ranges_overlap(const void* a, const void* b, uint64_t bytes) {
     const uintptr_t x = (uintptr_t)a, y = (uintptr_t)b;
     return x < y + bytes && y < x + bytes;

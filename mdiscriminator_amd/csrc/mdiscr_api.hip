@@ -324,25 +324,6 @@ constexpr int kQCR = (int)(kQTile / (kQCNW * kWave));
 static_assert(kQCR * kQCNW * kWave == (int)kQTile, "count and partition tiles agree");
 static_assert(kQTile >= kTile, "the workspace is sized by the smaller tile");
 
-// Dynamic LDS bytes that make `kernel` (static LDS as compiled) fit at most
-// `per_cu` workgroups per CU: the CU's LDS divided by per_cu, less the static part.
-template <typename K>
-size_t lds_pad_for(K kernel, int per_cu) {
-    if (per_cu <= 0) return 0;
-    hipFuncAttributes fa{};
-    int dev = 0, lds_cu = 0;
-    if (hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(kernel)) != hipSuccess ||
-        hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) !=
-            hipSuccess) {
-        (void)hipGetLastError();
-        return 0;
-    }
-    const size_t target = (size_t)lds_cu / (size_t)per_cu;
-    const size_t pad = target > fa.sharedSizeBytes ? target - fa.sharedSizeBytes : 0;
-    return pad & ~(size_t)255;
-}
-
 // The count and tile scan of a call in tiles of CNW * 64 * CR elements;
 // counts, prefix and totals are laid out in the workspace as above.
 template <int CNW, int CR, int NBITS, class Discr>

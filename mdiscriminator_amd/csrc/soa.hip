@@ -37,6 +37,12 @@ constexpr int kCT = 1;    // tiles per partition workgroup
 #ifndef MDISCR_SOA_NW
 #define MDISCR_SOA_NW 8
 #endif
+#ifndef MDISCR_SOA_WG_PER_CU
+#define MDISCR_SOA_WG_PER_CU 0
+#endif
+#ifndef MDISCR_SOA64_WG_PER_CU
+#define MDISCR_SOA64_WG_PER_CU 0
+#endif
 // The select class source (three gathered columns, typically 4 + 1 + 1 bytes)
 // at 8..32 classes: 4 waves x 12 rounds = 3072 elements.  Measured on 2^26
 // (uint, uchar, bool) records, m=8, interleaved A/B: count 0.0658 -> 0.0598 ms,
@@ -63,6 +69,8 @@ struct SoaGeom {
     // 2 / 8 tiles: m=16 0.042 / 0.045 / 0.051, select m=8 0.066 / 0.069 /
     // 0.079, m=64 0.050 / 0.046 / 0.048).
     static constexpr int CCT = kLarge ? 2 : 1;
+    // Partition workgroups resident per CU (0: as many as fit).
+    static constexpr int WGCU = kLarge ? MDISCR_SOA64_WG_PER_CU : MDISCR_SOA_WG_PER_CU;
 };
 
 // The tile launch_soa uses for these columns, classes and class source (the
@@ -122,9 +130,11 @@ mdiscr_status launch_soa(mdiscr_ctx* ctx, const SoaCols& cols, Src src, uint32_t
     timing_begin(ctx, s, tl);
     launch_tile_scan(counts, prefix, T, ld, m, totals, user_counts, nullptr, s);
     timing_end(ctx, s, tl, "soa_scan");
+    constexpr auto kern = k_partition_soa<NW, R, kCT, NBITS, Src, ST, Discr>;
+    static const size_t pad = lds_pad_for(kern, G::WGCU);
     timing_begin(ctx, s, tl);
-    hipLaunchKernelGGL((k_partition_soa<NW, R, kCT, NBITS, Src, ST, Discr>), dim3(Kp),
-                       dim3(NW * kWave), 0, s, cols, src, n, dis, prefix, totals, ld, m, nullptr);
+    hipLaunchKernelGGL(kern, dim3(Kp), dim3(NW * kWave), pad, s, cols, src, n, dis, prefix, totals,
+                       ld, m, nullptr);
     timing_end(ctx, s, tl, "soa_partition");
     return ctx_hip(ctx, hipGetLastError(), "SoA partition launch");
 }

@@ -27,10 +27,6 @@ namespace mdiscr {
 
 constexpr int kMaxCols = 8;
 
-#ifndef MDISCR_SOA_PIPELINE
-#define MDISCR_SOA_PIPELINE 0
-#endif
-
 struct SoaCols {
     const void* in[kMaxCols];
     void* out[kMaxCols];
@@ -324,73 +320,6 @@ __global__ __launch_bounds__(NW* kWave) void k_partition_soa(
         // 4. Every column through the staging buffer, the key column first:
         // its values are already in registers, which are then free for the
         // other columns' loads.
-#if MDISCR_SOA_PIPELINE
-        // 4-byte columns are software-pipelined: the next one's loads are
-        // issued before the current column's class runs are written, so its
-        // load latency overlaps those stores instead of following them.
-        auto next_col = [&](uint32_t j0) {
-            uint32_t j = j0;
-            if constexpr (Src::kKeyInRegs) {
-                if (j == cols.key) ++j;
-            }
-            return j;
-        };
-        uint32_t nx[R];
-        uint32_t jn = next_col(0);
-        bool have = jn < cols.ncols && cols.bytes[jn] == 4u;
-        // MDISCR_SOA_PIPELINE 2: the first column's loads go out before the
-        // keys are written (more registers live at once); 1: after.
-        if (MDISCR_SOA_PIPELINE == 2 && have)
-            gather_col<uint32_t, R>(static_cast<const uint32_t*>(cols.in[jn]), wbase + lane, kWave,
-                                    n, full, nx);
-        if constexpr (Src::kKeyInRegs)
-            soa_write_keys<typename Src::KeyT, NW, R>(v, cols.out[cols.key], slot, dest, valid,
-                                                      stage);
-        if (MDISCR_SOA_PIPELINE != 2 && have)
-            gather_col<uint32_t, R>(static_cast<const uint32_t*>(cols.in[jn]), wbase + lane, kWave,
-                                    n, full, nx);
-        while (jn < cols.ncols) {
-            const uint32_t j = jn;
-            jn = next_col(j + 1);
-            const bool had = have;
-            have = jn < cols.ncols && cols.bytes[jn] == 4u;
-            if (had) {
-                uint32_t* st = reinterpret_cast<uint32_t*>(stage);
-#pragma unroll
-                for (int r = 0; r < R; ++r) st[slot[r]] = nx[r];
-                if (have)
-                    gather_col<uint32_t, R>(static_cast<const uint32_t*>(cols.in[jn]), wbase + lane,
-                                            kWave, n, full, nx);
-                __syncthreads();
-                uint32_t* __restrict__ dst = static_cast<uint32_t*>(cols.out[j]);
-#pragma unroll
-                for (int k = 0; k < R; ++k) {
-                    const uint32_t s = k * THREADS + threadIdx.x;
-                    if (s < valid) __builtin_nontemporal_store(st[s], dst + dest.at(k, s));
-                }
-                __syncthreads();
-                continue;
-            }
-            switch (cols.bytes[j]) {
-                case 1:
-                    soa_move_column<uint8_t, NW, R>(cols.in[j], cols.out[j], wbase, n, full,
-                                                    slot, dest, valid, stage, lane);
-                    break;
-                case 2:
-                    soa_move_column<uint16_t, NW, R>(cols.in[j], cols.out[j], wbase, n, full,
-                                                     slot, dest, valid, stage, lane);
-                    break;
-                default:
-                    if constexpr (sizeof(ST) >= 8)
-                        soa_move_column<uint64_t, NW, R>(cols.in[j], cols.out[j], wbase, n, full,
-                                                         slot, dest, valid, stage, lane);
-                    break;
-            }
-            if (have)
-                gather_col<uint32_t, R>(static_cast<const uint32_t*>(cols.in[jn]), wbase + lane,
-                                        kWave, n, full, nx);
-        }
-#else
         if constexpr (Src::kKeyInRegs)
             soa_write_keys<typename Src::KeyT, NW, R>(v, cols.out[cols.key], slot, dest, valid,
                                                       stage);
@@ -418,7 +347,6 @@ __global__ __launch_bounds__(NW* kWave) void k_partition_soa(
                     break;
             }
         }
-#endif
     }
 }
 

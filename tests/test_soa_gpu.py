@@ -104,22 +104,25 @@ def test_soa_module_level_tuple_call(ctx):
     assert np.array_equal(host(o3, b), ref[2])
 
 
-def test_soa_full_size_2p26(ctx):
-    """Config-size run: 2^26 records, m=64, checked against the oracle."""
+@pytest.mark.parametrize("m", [16, 32, 64])
+def test_soa_full_size_2p26(ctx, m):
+    """Config-size run: 2^26 (key, f32 bits, index) records at each class
+    count of the configuration, checked against the oracle."""
     import torch
     import mdiscriminator_amd as md
     n = 1 << 26
     seed = 0x5EED0000 + 3
+    print(f"soa full size m={m}", flush=True)
     d_key = W.uniform_u31_torch(n, seed, "cuda:0")
     key = d_key.cpu().numpy().view(np.uint32)
     pay = W.normal_f32_bits(n, seed + 1)
     d_pay = dev(pay)
     d_idx = torch.arange(n, dtype=torch.int32, device="cuda:0")
-    outs = [torch.empty_like(d_key), torch.empty_like(d_pay), torch.empty_like(d_idx)]
-    d_cnt = torch.empty(64, dtype=torch.int32, device="cuda:0")
-    ctx.partition_soa([d_key, d_pay, d_idx], outs, md.Mask(64), d_counts=d_cnt)
+    outs = [torch.full_like(d_key, -1), torch.full_like(d_pay, -1), torch.full_like(d_idx, -1)]
+    d_cnt = torch.full((m,), -1, dtype=torch.int32, device="cuda:0")
+    ctx.partition_soa([d_key, d_pay, d_idx], outs, md.Mask(m), d_counts=d_cnt)
     torch.cuda.synchronize()
-    ref, ref_cnt = oracle.seq_mdiscr_soa([key, pay, np.arange(n, dtype=np.uint32)], 64,
+    ref, ref_cnt = oracle.seq_mdiscr_soa([key, pay, np.arange(n, dtype=np.uint32)], m,
                                          kind=oracle.MASK)
     assert np.array_equal(d_cnt.cpu().numpy().view(np.uint32), ref_cnt)
     for o, r in zip(outs, ref):
