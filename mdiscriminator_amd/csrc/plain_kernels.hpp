@@ -34,8 +34,18 @@
 #ifndef MDISCR_NT_LOAD
 #define MDISCR_NT_LOAD 1
 #endif
-#ifndef MDISCR_NT_STORE
-#define MDISCR_NT_STORE 1
+// Output stores of the quad-ranked partition (m <= 16) and of the
+// ballot-ranked one (1 = non-temporal hint).  A tile's class runs start and
+// end mid-line; with write-back stores the neighbouring tile, which runs on
+// the same XCD at about the same time, completes those lines in L2 before
+// they go to HBM, where a non-temporal store sends each part on its own.  So
+// both write back: 2^28 keys, m=32 0.482 -> 0.444 ms, m=64 0.526 -> 0.504,
+// m=2..16 1-2% faster.
+#ifndef MDISCR_QUAD_NT_STORE
+#define MDISCR_QUAD_NT_STORE 0
+#endif
+#ifndef MDISCR_BALLOT_NT_STORE
+#define MDISCR_BALLOT_NT_STORE 0
 #endif
 #ifndef MDISCR_COUNT_PREFETCH
 #define MDISCR_COUNT_PREFETCH 1
@@ -60,13 +70,11 @@ __device__ __forceinline__ T ld_stream(const T* p) {
     return *p;
 #endif
 }
-template <typename T>
-__device__ __forceinline__ void st_stream(T v, T* p) {
-#if MDISCR_NT_STORE
-    __builtin_nontemporal_store(v, p);
-#else
-    *p = v;
-#endif
+// NT: a non-temporal (streaming) store; else an ordinary write-back store.
+template <bool NT, typename T>
+__device__ __forceinline__ void st_out(T v, T* p) {
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
 }
 
 template <int NW, int R, int CT>
@@ -426,7 +434,7 @@ __device__ __forceinline__ void load_rounds(const uint32_t* __restrict__ in, uin
 
 // Phase 4: slot s = k*THREADS + tid goes to out[dpos[class] + s], in batches
 // of WB slots per thread (the LDS reads of a batch are issued together).
-template <int R, int THREADS, int WB, bool FULL, class Discr>
+template <int R, int THREADS, int WB, bool FULL, bool NT, class Discr>
 __device__ __forceinline__ void write_runs(const uint32_t* stage, const uint32_t* dpos,
                                            uint32_t* __restrict__ out, uint32_t valid,
                                            Discr dis) {
@@ -444,7 +452,7 @@ __device__ __forceinline__ void write_runs(const uint32_t* stage, const uint32_t
 #pragma unroll
         for (int k = 0; k < WB; ++k) {
             const uint32_t s = (k0 + k) * THREADS + threadIdx.x;
-            if (FULL || s < valid) st_stream(x[k], out + (d[k] + s));
+            if (FULL || s < valid) st_out<NT>(x[k], out + (d[k] + s));
         }
     }
 }
@@ -553,8 +561,8 @@ __global__ __launch_bounds__(NW* kWave, MINW) void k_partition(
         __syncthreads();
 
         // 4. Contiguous class runs to global memory.
-        if (full) write_runs<R, THREADS, WB, true>(stage, dpos, out, valid, dis);
-        else write_runs<R, THREADS, WB, false>(stage, dpos, out, valid, dis);
+        if (full) write_runs<R, THREADS, WB, true, MDISCR_BALLOT_NT_STORE>(stage, dpos, out, valid, dis);
+        else write_runs<R, THREADS, WB, false, MDISCR_BALLOT_NT_STORE>(stage, dpos, out, valid, dis);
 
         if (t + 1 < t1) {
             if (PREFETCH) {
@@ -807,8 +815,8 @@ __global__ __launch_bounds__(NW* kWave, MINW) void k_partition_quad(
 
     // 4. Class runs to global memory.
     if constexpr (!OUT16) {
-        if (full) write_runs<R, THREADS, WB, true>(stage, dpos, out, valid, dis);
-        else write_runs<R, THREADS, WB, false>(stage, dpos, out, valid, dis);
+        if (full) write_runs<R, THREADS, WB, true, MDISCR_QUAD_NT_STORE>(stage, dpos, out, valid, dis);
+        else write_runs<R, THREADS, WB, false, MDISCR_QUAD_NT_STORE>(stage, dpos, out, valid, dis);
     } else {
         constexpr int GROUPS = SLOTS / 4;
         const u32x4* st4 = reinterpret_cast<const u32x4*>(stage);
@@ -823,14 +831,15 @@ __global__ __launch_bounds__(NW* kWave, MINW) void k_partition_quad(
             const bool in0 = s0 - (m0 & 0xFFFFu) < (m0 >> 16);
             const bool in3 = s0 + 3u - (m3 & 0xFFFFu) < (m3 >> 16);
             if (in0 && in3 && d0 == d3) {
-                st_stream(x, reinterpret_cast<u32x4*>(out + (dpos[d0] + s0)));
+                st_out<MDISCR_QUAD_NT_STORE>(x, reinterpret_cast<u32x4*>(out + (dpos[d0] + s0)));
             } else {
                 const uint32_t xs[4] = {x.x, x.y, x.z, x.w};
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     const uint32_t d = dis(xs[j]);
                     const uint32_t mt = meta[d];
-                    if (s0 + j - (mt & 0xFFFFu) < (mt >> 16)) st_stream(xs[j], out + (dpos[d] + s0 + j));
+                    if (s0 + j - (mt & 0xFFFFu) < (mt >> 16))
+                        st_out<MDISCR_QUAD_NT_STORE>(xs[j], out + (dpos[d] + s0 + j));
                 }
             }
         }

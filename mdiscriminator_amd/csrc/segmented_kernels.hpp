@@ -23,6 +23,13 @@
 
 #include "plain_kernels.hpp"
 
+// Output stores of the segmented partition (1 = non-temporal hint).  Unlike
+// the single-array kernels, write-back stores were slower here: 10,000
+// lognormal segments, m=4, 0.315-0.321 -> 0.335-0.336 ms per call.
+#ifndef MDISCR_SEG_NT_STORE
+#define MDISCR_SEG_NT_STORE 1
+#endif
+
 namespace mdiscr {
 
 struct SegDesc {
@@ -442,14 +449,14 @@ __global__ __launch_bounds__(NW* kWave) void k_seg_partition(
                     const uint32_t c = packed[r] >> 16;
                     const uint32_t slot = cursor[wave][c] + (packed[r] & 0xFFFFu);
                     if (STAGED) st[slot] = v[r];
-                    else if (slot < d.len) __builtin_nontemporal_store(v[r], d.out + slot);
+                    else if (slot < d.len) st_out<MDISCR_SEG_NT_STORE>(v[r], d.out + slot);
                 }
                 if (STAGED) {
                     __builtin_amdgcn_wave_barrier();
 #pragma unroll
                     for (int k = 0; k < R; ++k) {
                         const uint32_t s = k * kWave + lane;
-                        if (s < d.len) __builtin_nontemporal_store(st[s], d.out + s);
+                        if (s < d.len) st_out<MDISCR_SEG_NT_STORE>(st[s], d.out + s);
                     }
                 }
             }
@@ -505,15 +512,15 @@ __global__ __launch_bounds__(NW* kWave) void k_seg_partition(
                 stage[cursor[wave][c] + (packed[r] & 0xFFFFu)] = v[r];
             }
             __syncthreads();
-            if (full) write_runs<R, THREADS, (R < 8 ? R : 8), true>(stage, dpos, d.out, valid, dis);
-            else write_runs<R, THREADS, (R < 8 ? R : 8), false>(stage, dpos, d.out, valid, dis);
+            if (full) write_runs<R, THREADS, (R < 8 ? R : 8), true, MDISCR_SEG_NT_STORE>(stage, dpos, d.out, valid, dis);
+            else write_runs<R, THREADS, (R < 8 ? R : 8), false, MDISCR_SEG_NT_STORE>(stage, dpos, d.out, valid, dis);
         } else {
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const uint32_t c = packed[r] >> 16;
                 const uint32_t slot = cursor[wave][c] + (packed[r] & 0xFFFFu);
                 if (full || slot < valid)
-                    __builtin_nontemporal_store(v[r], d.out + (dpos[c] + slot));
+                    st_out<MDISCR_SEG_NT_STORE>(v[r], d.out + (dpos[c] + slot));
             }
         }
     }
