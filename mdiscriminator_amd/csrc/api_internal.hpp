@@ -78,9 +78,9 @@ inline size_t lds_pad_for(K kernel, int per_cu) {
         (void)hipGetLastError();
         return 0;
     }
-    const size_t target = (size_t)lds_cu / (size_t)per_cu;
-    const size_t pad = target > fa.sharedSizeBytes ? target - fa.sharedSizeBytes : 0;
-    return pad & ~(size_t)255;
+    // The padded size is kept to a KiB multiple.
+    const size_t target = ((size_t)lds_cu / (size_t)per_cu) & ~(size_t)1023;
+    return target > fa.sharedSizeBytes ? target - fa.sharedSizeBytes : 0;
 }
 
 inline bool ranges_overlap(const void* a, const void* b, uint64_t bytes) {

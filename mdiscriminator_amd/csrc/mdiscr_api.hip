@@ -13,6 +13,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/mdiscr.h"
@@ -378,6 +379,66 @@ mdiscr_status launch_plain_quad(mdiscr_ctx* ctx, const uint32_t* in, uint32_t* o
     return ctx_hip(ctx, hipGetLastError(), "plain partition launch");
 }
 
+// The two-digit quad path (k_partition_lsd) on a 16-byte aligned input of
+// more than one of its tiles: NW waves x R keys per lane, WGCU workgroups
+// resident per CU (0: as many as fit, which is 4 by its 35 KB of LDS).
+// Unlike the one-pass quad kernel it wants every workgroup that fits: its
+// five barriers per tile leave a lone pair of workgroups idle.  2^28 keys,
+// partition ms at m = 64 and m = 32: 0.409 and 0.397 at 4 per CU, 0.544 and
+// 0.529 at 2 (3 per CU measured as 2); 8-wave tiles of 16 rounds 0.418 and
+// 0.400; 4096-key tiles 0.438 and 0.404; the ballot kernel 0.505 and 0.442.
+// MDISCR_PLAIN_LSD = 0 keeps the ballot-ranked kernel.
+#ifndef MDISCR_PLAIN_LSD
+#define MDISCR_PLAIN_LSD 1
+#endif
+#ifndef MDISCR_LSD_NW
+#define MDISCR_LSD_NW 4
+#endif
+#ifndef MDISCR_LSD_R
+#define MDISCR_LSD_R 32
+#endif
+#ifndef MDISCR_LSD_WG_PER_CU
+#define MDISCR_LSD_WG_PER_CU 0
+#endif
+
+// Where the two-digit path serves: 17..64 classes, and 16 classes under the
+// mask (2^28 keys: 0.412 -> 0.386 ms).  Under the modulus, at 9..16 classes,
+// the one-pass quad kernel stays: the two-digit kernel evaluates x % m once
+// per pass and once more for the write-out, which ate its gain (m = 9 / 13:
+// 0.463 / 0.463 ms one-pass against 0.466 / 0.470 ms).
+template <int NBITS, class Discr>
+constexpr bool use_lsd() {
+    return MDISCR_PLAIN_LSD != 0 &&
+           (NBITS >= 5 || (NBITS == 4 && std::is_same<Discr, DiscrMask>::value));
+}
+constexpr uint32_t kLsdTile = (uint32_t)MDISCR_LSD_NW * kWave * MDISCR_LSD_R;
+static_assert(kLsdTile >= kTile, "the workspace is sized by the smaller tile");
+
+template <int NBITS, class Discr>
+mdiscr_status launch_plain_lsd(mdiscr_ctx* ctx, const uint32_t* in, uint32_t* out, uint32_t n,
+                               Discr dis, uint32_t m, uint32_t* user_counts, uint64_t* offsets,
+                               hipStream_t s, uint32_t* ws) {
+    constexpr int NW = MDISCR_LSD_NW, R = MDISCR_LSD_R;
+    constexpr int CNW = 8;
+    constexpr int CR = (int)(kLsdTile / (CNW * kWave));
+    static_assert(CR * CNW * kWave == (int)kLsdTile, "count and partition tiles agree");
+    const uint32_t T = (uint32_t)(((uint64_t)n + kLsdTile - 1) / kLsdTile);
+    const uint32_t ld = (T + 3u) & ~3u;
+    uint32_t* counts = ws;
+    uint32_t* prefix = ws + (size_t)m * ld;
+    uint32_t* totals = ws + (size_t)2 * m * ld;
+    launch_count_scan<CNW, CR, NBITS>(ctx, in, n, dis, m, user_counts, s, counts, prefix, totals,
+                                      T, ld);
+    constexpr auto kern = k_partition_lsd<NW, R, NBITS, MDISCR_PLAIN_WB, MDISCR_PLAIN_MINW, Discr>;
+    static const size_t pad = lds_pad_for(kern, MDISCR_LSD_WG_PER_CU);
+    TimedLaunch tl;
+    timing_begin(ctx, s, tl);
+    hipLaunchKernelGGL(kern, dim3(T), dim3(NW * kWave), pad, s, in, out, n, dis, prefix, totals,
+                       ld, m, offsets);
+    timing_end(ctx, s, tl, "partition");
+    return ctx_hip(ctx, hipGetLastError(), "plain partition launch");
+}
+
 // Geometry of the ballot-ranked path from MDISCR_BIG_MIN_M classes: NW waves
 // x R rounds per tile, CNW count waves per tile, WGCU partition workgroups
 // per CU (0: as many as fit).  With many classes a 4096-key tile's class runs
@@ -444,6 +505,10 @@ template <int NBITS, class Discr>
 mdiscr_status launch_plain(mdiscr_ctx* ctx, const uint32_t* in, uint32_t* out, uint32_t n,
                            Discr dis, uint32_t m, uint32_t* user_counts, uint64_t* offsets,
                            hipStream_t s, uint32_t* ws) {
+    if constexpr (use_lsd<NBITS, Discr>()) {
+        if (((uintptr_t)in & 15u) == 0 && n > kLsdTile)
+            return launch_plain_lsd<NBITS>(ctx, in, out, n, dis, m, user_counts, offsets, s, ws);
+    }
     if constexpr (NBITS <= 4 && MDISCR_PLAIN_QUAD != 0) {
         if (((uintptr_t)in & 15u) == 0)
             return launch_plain_quad<NBITS>(ctx, in, out, n, dis, m, user_counts, offsets, s, ws);

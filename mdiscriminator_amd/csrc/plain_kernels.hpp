@@ -846,6 +846,158 @@ __global__ __launch_bounds__(NW* kWave, MINW) void k_partition_quad(
     }
 }
 
+// ---------------------------------------------------------------------------
+// The quad partition for 17..64 classes, as two stable digit passes inside
+// the tile (least significant first).  Packed byte counters rank at most 16
+// classes cheaply; 64 classes by ballots cost about 6 VALU per class bit per
+// element.  So the tile is ranked and staged in LDS by the class's low 3 bits
+// (8 digits), read back in that order, ranked and staged again by its high
+// bits (4 or 8 digits).  Both passes are stable, so LDS then holds the tile
+// grouped by class, each class in input order, and the class runs are
+// written as in the one-pass kernel.  A class's run starts inside the tile at
+// the sum of the tile's counts of the lower classes, which the count pass's
+// scan already holds: prefix[c][t + 1] - prefix[c][t] (totals[c] for t + 1).
+
+// Digit of the class: (class >> SHIFT) & MASKD.
+template <class Discr, int SHIFT, uint32_t MASKD>
+struct ClassDigit {
+    Discr dis;
+    __device__ __forceinline__ uint32_t operator()(uint32_t x) const {
+        return (dis(x) >> SHIFT) & MASKD;
+    }
+};
+
+// Wave 0, lane = digit d < DC: group sizes qb[w][q][d] (groups in memory
+// order) become LDS slot bases ordered by digit, then group.
+template <int NW, int Q, int DC>
+__device__ __forceinline__ void digit_bases(uint32_t (*qb)[Q][DC], uint32_t lane) {
+    uint32_t g[NW * Q];
+    uint32_t ct = 0;
+    if (lane < (uint32_t)DC) {
+#pragma unroll
+        for (int i = 0; i < NW * Q; ++i) {
+            const uint32_t x = qb[i / Q][i % Q][lane];
+            g[i] = ct;
+            ct += x;
+        }
+    }
+    const uint32_t start = wave_excl_scan(ct, lane);
+    if (lane < (uint32_t)DC) {
+#pragma unroll
+        for (int i = 0; i < NW * Q; ++i) qb[i / Q][i % Q][lane] = g[i] + start;
+    }
+}
+
+// Stage quad q's four elements at qb[q][digit] + rank (digits and ranks in
+// bytes j of ck / rk).
+template <int Q, int DC>
+__device__ __forceinline__ void stage_quads(uint32_t* stage, const u32x4 (&v)[Q],
+                                            const uint32_t (&ck)[Q], const uint32_t (&rk)[Q],
+                                            const uint32_t (*qb)[DC]) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const uint32_t x[4] = {v[q].x, v[q].y, v[q].z, v[q].w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t c = (ck[q] >> (8 * j)) & 0xFFu;
+            const uint32_t r = (rk[q] >> (8 * j)) & 0xFFu;
+            stage[qb[q][c] + r] = x[j];
+        }
+    }
+}
+
+// Tiles of NW waves x R keys per lane (quads as in k_partition_quad), on a
+// 16-byte aligned input of at least two tiles (the count pass ran).  Keys past
+// n are given the value m - 1, whose class is m - 1 for the mask and the
+// modulus alike: both passes then rank them after every real key of class
+// m - 1, in the LDS slots at or past `valid`, which are never written out.
+template <int NW, int R, int NBITS, int WB, int MINW, class Discr>
+__global__ __launch_bounds__(NW* kWave, MINW) void k_partition_lsd(
+    const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t n, Discr dis,
+    const uint32_t* __restrict__ prefix, const uint32_t* __restrict__ totals, uint32_t ld,
+    uint32_t m, uint64_t* __restrict__ offsets) {
+    static_assert(NBITS > 3 && NBITS <= 6, "two digits: 3 low bits, 1..3 high bits");
+    constexpr int LB = 3, HB = NBITS - 3;
+    constexpr int LC = 1 << LB, HC = 1 << HB;
+    constexpr int Q = R / 4;
+    constexpr int THREADS = NW * kWave;
+    constexpr int TILE = THREADS * R;
+    constexpr uint32_t WT = kWave * R;
+    static_assert(R % 4 == 0, "whole quads per lane");
+    __shared__ __attribute__((aligned(16))) uint32_t stage[TILE];
+    __shared__ uint32_t qlo[NW][Q][LC];  // group sizes, then slot bases: low digit
+    __shared__ uint32_t qhi[NW][Q][HC];  // the same for the high digit
+    __shared__ uint32_t dpos[1 << NBITS];  // output index - LDS slot, per class
+
+    const uint32_t lane = lane_id();
+    const uint32_t wave = wave_id();
+    const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t tbase = t * (uint32_t)TILE;
+    const uint32_t wbase = tbase + wave * WT;
+    const bool full = (uint64_t)tbase + TILE <= n;
+    const uint32_t valid = full ? (uint32_t)TILE : n - tbase;
+
+    // Class offsets of this tile (wave 0, lane = class), issued first.
+    uint32_t pre = 0, nxt = 0, tot = 0;
+    if (wave == 0 && lane < m) {
+        const uint32_t* pr = prefix + (size_t)lane * ld;
+        pre = pr[t];
+        tot = totals[lane];
+        nxt = t + 1u < gridDim.x ? pr[t + 1u] : tot;
+    }
+
+    u32x4 v[Q];
+    if (full) {
+        const u32x4* lp = reinterpret_cast<const u32x4*>(in + wbase) + lane;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) v[q] = ld_stream(lp + q * kWave);
+    } else {
+        const uint32_t fill = m - 1u;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const uint32_t e = wbase + q * (kWave * 4) + lane * 4;
+            v[q].x = e < n ? in[e] : fill;
+            v[q].y = e + 1 < n ? in[e + 1] : fill;
+            v[q].z = e + 2 < n ? in[e + 2] : fill;
+            v[q].w = e + 3 < n ? in[e + 3] : fill;
+        }
+    }
+
+    // Pass 1: rank by the low digit, stage in (digit, memory) order.
+    uint32_t ck[Q], rk[Q];
+    quad_rank<Q, LB, true>(v, ck, rk, qlo[wave], 0u, 0u, m, lane,
+                           ClassDigit<Discr, 0, LC - 1>{dis});
+    __syncthreads();
+    if (wave == 0) {
+        digit_bases<NW, Q, LC>(qlo, lane);
+        const uint32_t running = wave_excl_scan(tot, lane) + pre;
+        const uint32_t cstart = wave_excl_scan(nxt - pre, lane);  // class start in the tile
+        if (lane < (1u << NBITS)) dpos[lane] = running - cstart;
+        if (offsets && t == 0) {
+            if (lane < m) offsets[lane] = running;
+            if (lane == 0) offsets[m] = n;
+        }
+    }
+    __syncthreads();
+    stage_quads<Q, LC>(stage, v, ck, rk, qlo[wave]);
+    __syncthreads();
+
+    // Pass 2: the wave's share of that order, ranked by the high digit.
+    const u32x4* sp = reinterpret_cast<const u32x4*>(stage + wave * WT) + lane;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) v[q] = sp[q * kWave];
+    quad_rank<Q, HB, true>(v, ck, rk, qhi[wave], 0u, 0u, m, lane,
+                           ClassDigit<Discr, LB, HC - 1>{dis});
+    __syncthreads();  // every wave has read its share of stage
+    if (wave == 0) digit_bases<NW, Q, HC>(qhi, lane);
+    __syncthreads();
+    stage_quads<Q, HC>(stage, v, ck, rk, qhi[wave]);
+    __syncthreads();
+
+    if (full) write_runs<R, THREADS, WB, true, MDISCR_QUAD_NT_STORE>(stage, dpos, out, valid, dis);
+    else write_runs<R, THREADS, WB, false, MDISCR_QUAD_NT_STORE>(stage, dpos, out, valid, dis);
+}
+
 // d_counts[c] = 0 for n == 0, and other small fills (a kernel rather than a
 // memset: the graphs this library captures hold kernel nodes only).
 template <int = 0>

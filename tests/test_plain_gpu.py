@@ -220,6 +220,33 @@ def test_full_size_2p28_m8(ctx, variant):
     assert i < 0, f"first mismatch at {i}"
 
 
+@pytest.mark.parametrize("discr", ["mask64", "mod33", "mod47"])
+def test_full_size_2p28_many_classes(ctx, discr):
+    """2^28 int32 over 33..64 classes (the two-digit path: low 3 class bits,
+    then the high bits, both ranked in LDS), bit-exact against the oracle,
+    with the class offsets."""
+    import torch
+    import mdiscriminator_amd as md
+    n = 1 << 28
+    m = int(discr[4:] if discr.startswith("mask") else discr[3:])
+    d, kind = (md.Mask(m), oracle.MASK) if discr.startswith("mask") else (md.Mod(m), oracle.MOD)
+    print(f"2^28 {discr}: generating", flush=True)
+    d_in = W.uniform_u31_torch(n, W.SEED_BASE + 3 + m, "cuda:0")
+    d_out = torch.empty_like(d_in)
+    d_cnt = torch.full((m,), -1, dtype=torch.int32, device="cuda:0")
+    d_off = torch.full((m + 1,), -1, dtype=torch.int64, device="cuda:0")
+    ctx.partition_u32_ex(d_in, d_out, d, counts=d_cnt, offsets=d_off)
+    torch.cuda.synchronize()
+    x = from_dev(d_in)
+    ref, ref_cnt = oracle.seq_mdiscr(x, m, kind)
+    assert np.array_equal(from_dev(d_cnt), ref_cnt)
+    off = d_off.cpu().numpy()
+    assert off[0] == 0 and off[m] == n
+    assert np.array_equal(np.diff(off), ref_cnt.astype(np.int64))
+    i = first_diff(from_dev(d_out), ref)
+    assert i < 0, f"{discr}: first mismatch at {i}"
+
+
 @pytest.mark.parametrize("m", [8, 7])
 def test_maximum_size(ctx, m):
     """n = MDISCR_MAX_ELEMS (0xFFFF0000, about 4.29 G elements, 17 GB each way)
