@@ -15,24 +15,26 @@ using namespace mdiscr;
 namespace {
 
 // Tile geometry by staging width and class bits (measured on 2^26
-// three-column records, interleaved A/B).  8-byte staging (a column wider
-// than 4 bytes): 8 waves x 8 rounds = 4096 elements.  4-byte staging: 8 waves
-// x 8 rounds up to 4 classes; 8 waves x 12 rounds = 6144 elements for 8..32
-// classes (m=16: -1.6%, m=32: -2.1%, select at m=8: -11%; m=4 is 2.9% faster
-// at 8 rounds); 16 waves x 12 rounds = 12288 elements for 64 classes (8
-// rounds 0.416 ms, 12 rounds 0.403 ms, 16 rounds 0.426 ms; 8 waves x 12 or 16
-// rounds 0.465 / 0.497 ms against 0.406 ms in a later A/B).  16 rounds at 8
-// waves loses 20-30% (occupancy).
+// three-column records, interleaved A/B, with write-back column stores).
+// 8-byte staging (a column wider than 4 bytes): 8 waves x 8 rounds = 4096
+// elements.  4-byte staging: 8 waves x 8 rounds = 4096 elements up to 32
+// classes (against 12 rounds: m=8 0.297 -> 0.292 ms, m=16 0.295 -> 0.289,
+// m=32 0.306 -> 0.301); 8 waves x 12 rounds = 6144 elements for 64 classes
+// (0.320 ms; 16 waves x 12: 0.406, 16 x 8: 0.417, 8 x 8: 0.340, 4 x 12:
+// 0.330).  The 16-wave tile of 12288 won only while the stores were
+// non-temporal, when the partial lines at the ends of shorter class runs cost
+// HBM bytes; with write-back stores the L2 joins them, and two workgroups
+// per CU overlap each other's load, stage and write phases.
 constexpr int kCT = 1;    // tiles per partition workgroup
 
 #ifndef MDISCR_SOA_R
-#define MDISCR_SOA_R 12
+#define MDISCR_SOA_R 8
 #endif
 #ifndef MDISCR_SOA64_R
 #define MDISCR_SOA64_R 12
 #endif
 #ifndef MDISCR_SOA64_NW
-#define MDISCR_SOA64_NW 16
+#define MDISCR_SOA64_NW 8
 #endif
 #ifndef MDISCR_SOA_NW
 #define MDISCR_SOA_NW 8
@@ -44,7 +46,8 @@ constexpr int kCT = 1;    // tiles per partition workgroup
 #define MDISCR_SOA64_WG_PER_CU 0
 #endif
 // The select class source (three gathered columns, typically 4 + 1 + 1 bytes)
-// at 8..32 classes: 4 waves x 12 rounds = 3072 elements.  Measured on 2^26
+// at 8..64 classes: 4 waves x 12 rounds = 3072 elements (m=64: 0.321 ms with
+// the 6144-element tile of 4-byte keys, 0.268 ms with this one).  Measured on 2^26
 // (uint, uchar, bool) records, m=8, interleaved A/B: count 0.0658 -> 0.0598 ms,
 // partition 0.2862 -> 0.2669 ms against 8 waves (4 x 16: 0.2918 ms).  The same
 // change slows 4-byte key columns (m=16 partition 0.323 -> 0.347 ms).  With the
@@ -61,7 +64,7 @@ struct SoaGeom {
     static constexpr bool kLarge = sizeof(ST) == 4 && NBITS >= 6;
     static constexpr bool kMid = sizeof(ST) == 4 && NBITS >= 3 && NBITS < 6;
     static constexpr int NW =
-        kLarge ? MDISCR_SOA64_NW : (SEL && kMid ? MDISCR_SEL_NW : MDISCR_SOA_NW);
+        SEL && (kMid || kLarge) ? MDISCR_SEL_NW : (kLarge ? MDISCR_SOA64_NW : MDISCR_SOA_NW);
     static constexpr int R =
         kLarge ? MDISCR_SOA64_R : (kMid ? (SEL ? MDISCR_SEL_R : MDISCR_SOA_R) : 8);
     static constexpr uint32_t TILE = NW * kWave * R;
@@ -77,7 +80,7 @@ struct SoaGeom {
 // workspace size).
 uint32_t soa_tile(bool wide, uint32_t nbits, bool sel) {
     if (wide) return SoaGeom<uint64_t, 0>::TILE;
-    if (nbits >= 6) return SoaGeom<uint32_t, 6>::TILE;
+    if (nbits >= 6) return sel ? SoaGeom<uint32_t, 6, true>::TILE : SoaGeom<uint32_t, 6>::TILE;
     if (nbits >= 3) return sel ? SoaGeom<uint32_t, 3, true>::TILE : SoaGeom<uint32_t, 3>::TILE;
     return SoaGeom<uint32_t, 0>::TILE;
 }
@@ -85,7 +88,6 @@ static_assert(SoaGeom<uint32_t, 5>::TILE == SoaGeom<uint32_t, 3>::TILE &&
                   SoaGeom<uint32_t, 5, true>::TILE == SoaGeom<uint32_t, 3, true>::TILE &&
                   SoaGeom<uint32_t, 2>::TILE == SoaGeom<uint32_t, 0>::TILE &&
                   SoaGeom<uint32_t, 2, true>::TILE == SoaGeom<uint32_t, 0>::TILE &&
-                  SoaGeom<uint32_t, 6, true>::TILE == SoaGeom<uint32_t, 6>::TILE &&
                   SoaGeom<uint64_t, 6>::TILE == SoaGeom<uint64_t, 0>::TILE &&
                   SoaGeom<uint64_t, 3, true>::TILE == SoaGeom<uint64_t, 0>::TILE,
               "soa_tile covers every geometry");
