@@ -51,11 +51,19 @@ namespace {
 #ifndef MDISCR_ROWS_NT_LOAD
 #define MDISCR_ROWS_NT_LOAD 0
 #endif
+// Words per thread for 2-word rows: half the general figure (2^25 rows, m=16,
+// interleaved A/B with write-back stores: call 0.145 -> 0.132 ms, the count in
+// the same tiles 0.054 -> 0.046; 3-, 4- and 8-word rows were 1-11% slower
+// with fewer words per thread).
+#ifndef MDISCR_ROWS_WPT_LEN2
+#define MDISCR_ROWS_WPT_LEN2 16
+#endif
 constexpr int kNW = MDISCR_ROWS_NW;  // waves per workgroup
 // Rounds per lane: about MDISCR_ROWS_WPT words per thread, so a tile's rows
 // take NW * 64 * WPT * 4 bytes of LDS (32 KiB at the defaults).
 constexpr int rounds_for(int len) {
-    return MDISCR_ROWS_WPT / len < 2 ? 2 : MDISCR_ROWS_WPT / len;
+    const int wpt = len == 2 ? MDISCR_ROWS_WPT_LEN2 : MDISCR_ROWS_WPT;
+    return wpt / len < 2 ? 2 : wpt / len;
 }
 
 template <int NW, int R, int LEN, int NBITS, class Discr, bool SINGLE = false>

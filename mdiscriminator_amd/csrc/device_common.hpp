@@ -44,6 +44,19 @@ struct DiscrPred {
     }
 };
 
+// Discriminators whose evaluation costs more than an LDS byte read per
+// element (x % m: two quarter-rate 32-bit multiplies).  Kernels that group a
+// tile by class in LDS keep each slot's class there for the write-out instead
+// of evaluating them again.
+template <class D>
+struct CostlyDiscr {
+    static constexpr bool value = false;
+};
+template <>
+struct CostlyDiscr<DiscrMod> {
+    static constexpr bool value = true;
+};
+
 // The loaded value already is the class (class sources that apply their own
 // discriminator, e.g. a selection between two key columns).
 struct DiscrIdent {

@@ -44,6 +44,15 @@
 #ifndef MDISCR_QUAD_NT_STORE
 #define MDISCR_QUAD_NT_STORE 0
 #endif
+// The one-pass quad kernel under a costly discriminator (x % m) keeps each LDS
+// slot's class as a byte for the write-out instead of evaluating x % m again
+// there (2^28 keys: m=7 0.416 -> 0.397 ms, m=13 0.472 -> 0.458).  The
+// two-digit kernel does not: its 8 KB of class bytes leave room for three
+// workgroups per CU instead of four, and it got slower (m=33 0.504 -> 0.521,
+// m=48 0.506 -> 0.531).
+#ifndef MDISCR_QUAD_CLS
+#define MDISCR_QUAD_CLS 1
+#endif
 #ifndef MDISCR_BALLOT_NT_STORE
 #define MDISCR_BALLOT_NT_STORE 0
 #endif
@@ -457,6 +466,32 @@ __device__ __forceinline__ void write_runs(const uint32_t* stage, const uint32_t
     }
 }
 
+// write_runs with each slot's class read from cls (bytes in LDS) instead of
+// evaluated from the key.
+template <int R, int THREADS, int WB, bool FULL, bool NT>
+__device__ __forceinline__ void write_runs_cls(const uint32_t* stage, const uint8_t* cls,
+                                               const uint32_t* dpos, uint32_t* __restrict__ out,
+                                               uint32_t valid) {
+    static_assert(R % WB == 0, "write batch");
+#pragma unroll
+    for (int k0 = 0; k0 < R; k0 += WB) {
+        uint32_t x[WB], d[WB];
+#pragma unroll
+        for (int k = 0; k < WB; ++k) {
+            const uint32_t s = (k0 + k) * THREADS + threadIdx.x;
+            x[k] = (FULL || s < valid) ? stage[s] : 0u;
+            d[k] = (FULL || s < valid) ? (uint32_t)cls[s] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < WB; ++k) d[k] = dpos[d[k]];
+#pragma unroll
+        for (int k = 0; k < WB; ++k) {
+            const uint32_t s = (k0 + k) * THREADS + threadIdx.x;
+            if (FULL || s < valid) st_out<NT>(x[k], out + (d[k] + s));
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // partition: each workgroup runs its chunk's tiles in order, carrying the
 // running global offset of every class across tiles (wave 0, lane c).
@@ -720,6 +755,8 @@ __global__ __launch_bounds__(NW* kWave, MINW) void k_partition_quad(
     __shared__ uint32_t qbase[NW][Q][MC];  // group sizes, then LDS slot bases
     __shared__ uint32_t dpos[MC];          // output index - LDS slot, per class
     __shared__ uint32_t meta[MC];          // OUT16: run start | valid run length << 16
+    constexpr bool CLS = MDISCR_QUAD_CLS && CostlyDiscr<Discr>::value && !OUT16;
+    __shared__ uint8_t cls[CLS ? TILE : 1];  // class of each LDS slot
 
     const uint32_t lane = lane_id();
     const uint32_t wave = wave_id();
@@ -808,13 +845,18 @@ __global__ __launch_bounds__(NW* kWave, MINW) void k_partition_quad(
         for (int j = 0; j < 4; ++j) {
             const uint32_t c = (ck[q] >> (8 * j)) & 0xFFu;
             const uint32_t r = (rk[q] >> (8 * j)) & 0xFFu;
-            stage[qbase[wave][q][c] + r] = x[j];
+            const uint32_t sl = qbase[wave][q][c] + r;
+            stage[sl] = x[j];
+            if constexpr (CLS) cls[sl] = (uint8_t)c;
         }
     }
     __syncthreads();
 
     // 4. Class runs to global memory.
-    if constexpr (!OUT16) {
+    if constexpr (CLS) {
+        if (full) write_runs_cls<R, THREADS, WB, true, MDISCR_QUAD_NT_STORE>(stage, cls, dpos, out, valid);
+        else write_runs_cls<R, THREADS, WB, false, MDISCR_QUAD_NT_STORE>(stage, cls, dpos, out, valid);
+    } else if constexpr (!OUT16) {
         if (full) write_runs<R, THREADS, WB, true, MDISCR_QUAD_NT_STORE>(stage, dpos, out, valid, dis);
         else write_runs<R, THREADS, WB, false, MDISCR_QUAD_NT_STORE>(stage, dpos, out, valid, dis);
     } else {

@@ -4,7 +4,7 @@ launch that counts its own tile, one more takes the full count, scan and
 partition.  Class counts start filled with -1, so a count the call did not
 write shows up.  Tiles: SoA 4096 (m <= 32, 8-byte columns), 6144 (64
 classes); the select source 3072 (8..64 classes); array
-rows 4096 / 2560 / 1536 / 1024 elements for LEN 2 / 3 / 5 / 8."""
+rows 2048 / 2560 / 1536 / 1024 elements for LEN 2 / 3 / 5 / 8."""
 import numpy as np
 import pytest
 
@@ -108,7 +108,7 @@ def test_filter_records_one_tile(ctx):
         check_cols(outs, ref_outs, cnt, ref_cnt, f"filter records n={n}")
 
 
-@pytest.mark.parametrize("length,tile", [(2, 4096), (3, 2560), (5, 1536), (8, 1024)])
+@pytest.mark.parametrize("length,tile", [(2, 2048), (3, 2560), (5, 1536), (8, 1024)])
 def test_arrays_one_tile(ctx, length, tile):
     import torch
     import mdiscriminator_amd as md
