@@ -56,6 +56,10 @@ constexpr int kCT = 1;    // tiles per partition workgroup
 #ifndef MDISCR_SEL_NW
 #define MDISCR_SEL_NW 4
 #endif
+// The select geometry also below 8 classes (else the SoA one: 8 x 8).
+#ifndef MDISCR_SEL_ALL_M
+#define MDISCR_SEL_ALL_M 1
+#endif
 #ifndef MDISCR_SEL_R
 #define MDISCR_SEL_R 12
 #endif
@@ -63,10 +67,11 @@ template <typename ST, int NBITS, bool SEL = false>
 struct SoaGeom {
     static constexpr bool kLarge = sizeof(ST) == 4 && NBITS >= 6;
     static constexpr bool kMid = sizeof(ST) == 4 && NBITS >= 3 && NBITS < 6;
+    static constexpr bool kSel = SEL && sizeof(ST) == 4 && (MDISCR_SEL_ALL_M || NBITS >= 3);
     static constexpr int NW =
-        SEL && (kMid || kLarge) ? MDISCR_SEL_NW : (kLarge ? MDISCR_SOA64_NW : MDISCR_SOA_NW);
+        kSel ? MDISCR_SEL_NW : (kLarge ? MDISCR_SOA64_NW : MDISCR_SOA_NW);
     static constexpr int R =
-        kLarge ? MDISCR_SOA64_R : (kMid ? (SEL ? MDISCR_SEL_R : MDISCR_SOA_R) : 8);
+        kSel ? MDISCR_SEL_R : (kLarge ? MDISCR_SOA64_R : (kMid ? MDISCR_SOA_R : 8));
     static constexpr uint32_t TILE = NW * kWave * R;
     // Tiles per count workgroup (2^26 keys, interleaved A/B, count ms for 1 /
     // 2 / 8 tiles: m=16 0.042 / 0.045 / 0.051, select m=8 0.066 / 0.069 /
@@ -82,12 +87,12 @@ uint32_t soa_tile(bool wide, uint32_t nbits, bool sel) {
     if (wide) return SoaGeom<uint64_t, 0>::TILE;
     if (nbits >= 6) return sel ? SoaGeom<uint32_t, 6, true>::TILE : SoaGeom<uint32_t, 6>::TILE;
     if (nbits >= 3) return sel ? SoaGeom<uint32_t, 3, true>::TILE : SoaGeom<uint32_t, 3>::TILE;
-    return SoaGeom<uint32_t, 0>::TILE;
+    return sel ? SoaGeom<uint32_t, 0, true>::TILE : SoaGeom<uint32_t, 0>::TILE;
 }
 static_assert(SoaGeom<uint32_t, 5>::TILE == SoaGeom<uint32_t, 3>::TILE &&
                   SoaGeom<uint32_t, 5, true>::TILE == SoaGeom<uint32_t, 3, true>::TILE &&
                   SoaGeom<uint32_t, 2>::TILE == SoaGeom<uint32_t, 0>::TILE &&
-                  SoaGeom<uint32_t, 2, true>::TILE == SoaGeom<uint32_t, 0>::TILE &&
+                  SoaGeom<uint32_t, 2, true>::TILE == SoaGeom<uint32_t, 0, true>::TILE &&
                   SoaGeom<uint64_t, 6>::TILE == SoaGeom<uint64_t, 0>::TILE &&
                   SoaGeom<uint64_t, 3, true>::TILE == SoaGeom<uint64_t, 0>::TILE,
               "soa_tile covers every geometry");
@@ -196,7 +201,7 @@ mdiscr_status run_staged(mdiscr_ctx* ctx, const SoaCols& cols, uint32_t n, const
             return dispatch_discr<ST>(
                 ctx, cols,
                 SelectSrc{cols.in[cs.sel], cols.in[cs.a], cols.in[cs.b], cols.bytes[cs.sel] == 4u,
-                          cols.bytes[cs.a] == 4u, cols.bytes[cs.b] == 4u},
+                          cols.bytes[cs.a] == 4u, cols.bytes[cs.b] == 4u, cs.sel, cs.a, cs.b},
                 n, cs.d, d_counts, s, w);
     }
     return ctx_fail(ctx, MDISCR_EINVAL, "class kind");

@@ -100,6 +100,7 @@ struct SelectSrc {
     const void* a;
     const void* b;
     uint32_t sel4, a4, b4;  // nonzero when that column holds 4-byte values
+    uint32_t sel_col, a_col, b_col;  // their indices among the moved columns
     template <int R>
     __device__ __forceinline__ static void col(const void* p, uint32_t w4, uint32_t first,
                                                uint32_t step, uint32_t n, bool full,
@@ -117,7 +118,24 @@ struct SelectSrc {
 #pragma unroll
         for (int r = 0; r < R; ++r) v[r] = z[r] ? v[r] : y[r];
     }
+    // gather, keeping the three columns' values (zero-extended) as well.
+    template <int R>
+    __device__ __forceinline__ void gather_keep(uint32_t first, uint32_t step, uint32_t n,
+                                                bool full, uint32_t (&v)[R], uint32_t (&z)[R],
+                                                uint32_t (&x)[R], uint32_t (&y)[R]) const {
+        col<R>(sel, sel4, first, step, n, full, z);
+        col<R>(a, a4, first, step, n, full, x);
+        col<R>(b, b4, first, step, n, full, y);
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = z[r] ? x[r] : y[r];
+    }
 };
+
+// The select source's partition writes its three columns from the values it
+// loaded for the classes instead of loading them again.
+#ifndef MDISCR_SEL_KEEP
+#define MDISCR_SEL_KEEP 1
+#endif
 
 // ---------------------------------------------------------------------------
 // count pass over a class source (a strided or 1-byte key column, or a select).
@@ -274,7 +292,11 @@ __global__ __launch_bounds__(NW* kWave) void k_partition_soa(
         const uint32_t valid = full ? (uint32_t)TILE : n - tbase;
 
         uint32_t v[R];
-        if constexpr (Src::kKeyInRegs)  // the partition's key column is contiguous
+        constexpr bool kSelKeep = MDISCR_SEL_KEEP && std::is_same_v<Src, SelectSrc>;
+        [[maybe_unused]] uint32_t sz[kSelKeep ? R : 1], sa[kSelKeep ? R : 1], sb[kSelKeep ? R : 1];
+        if constexpr (kSelKeep)
+            src.template gather_keep<R>(wbase + lane, kWave, n, full, v, sz, sa, sb);
+        else if constexpr (Src::kKeyInRegs)  // the partition's key column is contiguous
             gather_col<typename Src::KeyT, R>(src.key, wbase + lane, kWave, n, full, v);
         else
             src.template gather<R>(wbase + lane, kWave, n, full, v);
@@ -331,9 +353,23 @@ __global__ __launch_bounds__(NW* kWave) void k_partition_soa(
         if constexpr (Src::kKeyInRegs)
             soa_write_keys<typename Src::KeyT, NW, R>(v, cols.out[cols.key], slot, dest, valid,
                                                       stage);
+        if constexpr (kSelKeep) {
+            // The select's columns from registers (a column named twice is
+            // written once).
+            auto put = [&](uint32_t j, uint32_t w4, const uint32_t (&x)[R]) {
+                if (w4) soa_write_keys<uint32_t, NW, R>(x, cols.out[j], slot, dest, valid, stage);
+                else soa_write_keys<uint8_t, NW, R>(x, cols.out[j], slot, dest, valid, stage);
+            };
+            put(src.a_col, src.a4, sa);
+            if (src.b_col != src.a_col) put(src.b_col, src.b4, sb);
+            if (src.sel_col != src.a_col && src.sel_col != src.b_col) put(src.sel_col, src.sel4, sz);
+        }
         for (uint32_t j = 0; j < cols.ncols; ++j) {
             if constexpr (Src::kKeyInRegs) {
                 if (j == cols.key) continue;
+            }
+            if constexpr (kSelKeep) {
+                if (j == src.a_col || j == src.b_col || j == src.sel_col) continue;
             }
             switch (cols.bytes[j]) {
                 case 1:
