@@ -394,6 +394,15 @@ mdiscr_status launch_plain_quad(mdiscr_ctx* ctx, const uint32_t* in, uint32_t* o
 #ifndef MDISCR_LSD_NW
 #define MDISCR_LSD_NW 4
 #endif
+// Under x % m the two-digit kernel carries each key's class from its first
+// pass to the second and to the write-out as a byte in LDS (x % m once per
+// key instead of three times), in tiles of MDISCR_LSD_CLS_R keys per lane.
+#ifndef MDISCR_LSD_CLS
+#define MDISCR_LSD_CLS 1
+#endif
+#ifndef MDISCR_LSD_CLS_R
+#define MDISCR_LSD_CLS_R 24
+#endif
 #ifndef MDISCR_LSD_R
 #define MDISCR_LSD_R 32
 #endif
@@ -406,30 +415,45 @@ mdiscr_status launch_plain_quad(mdiscr_ctx* ctx, const uint32_t* in, uint32_t* o
 // the one-pass quad kernel stays: the two-digit kernel evaluates x % m once
 // per pass and once more for the write-out, which ate its gain (m = 9 / 13:
 // 0.463 / 0.463 ms one-pass against 0.466 / 0.470 ms).
+#ifndef MDISCR_LSD_MOD16
+#define MDISCR_LSD_MOD16 1
+#endif
 template <int NBITS, class Discr>
 constexpr bool use_lsd() {
     return MDISCR_PLAIN_LSD != 0 &&
-           (NBITS >= 5 || (NBITS == 4 && std::is_same<Discr, DiscrMask>::value));
+           (NBITS >= 5 ||
+            (NBITS == 4 && (std::is_same<Discr, DiscrMask>::value ||
+                            (MDISCR_LSD_MOD16 && MDISCR_LSD_CLS && CostlyDiscr<Discr>::value))));
 }
-constexpr uint32_t kLsdTile = (uint32_t)MDISCR_LSD_NW * kWave * MDISCR_LSD_R;
-static_assert(kLsdTile >= kTile, "the workspace is sized by the smaller tile");
+// The two-digit kernel's geometry for a discriminator.
+template <class Discr>
+struct LsdGeom {
+    static constexpr bool CLS = MDISCR_LSD_CLS && CostlyDiscr<Discr>::value;
+    static constexpr int NW = MDISCR_LSD_NW;
+    static constexpr int R = CLS ? MDISCR_LSD_CLS_R : MDISCR_LSD_R;
+    static constexpr uint32_t TILE = (uint32_t)NW * kWave * R;
+    static_assert(TILE >= kTile, "the workspace is sized by the smaller tile");
+};
 
 template <int NBITS, class Discr>
 mdiscr_status launch_plain_lsd(mdiscr_ctx* ctx, const uint32_t* in, uint32_t* out, uint32_t n,
                                Discr dis, uint32_t m, uint32_t* user_counts, uint64_t* offsets,
                                hipStream_t s, uint32_t* ws) {
-    constexpr int NW = MDISCR_LSD_NW, R = MDISCR_LSD_R;
+    using G = LsdGeom<Discr>;
+    constexpr int NW = G::NW, R = G::R;
+    constexpr uint32_t TILE = G::TILE;
     constexpr int CNW = 8;
-    constexpr int CR = (int)(kLsdTile / (CNW * kWave));
-    static_assert(CR * CNW * kWave == (int)kLsdTile, "count and partition tiles agree");
-    const uint32_t T = (uint32_t)(((uint64_t)n + kLsdTile - 1) / kLsdTile);
+    constexpr int CR = (int)(TILE / (CNW * kWave));
+    static_assert(CR * CNW * kWave == (int)TILE, "count and partition tiles agree");
+    const uint32_t T = (uint32_t)(((uint64_t)n + TILE - 1) / TILE);
     const uint32_t ld = (T + 3u) & ~3u;
     uint32_t* counts = ws;
     uint32_t* prefix = ws + (size_t)m * ld;
     uint32_t* totals = ws + (size_t)2 * m * ld;
     launch_count_scan<CNW, CR, NBITS>(ctx, in, n, dis, m, user_counts, s, counts, prefix, totals,
                                       T, ld);
-    constexpr auto kern = k_partition_lsd<NW, R, NBITS, MDISCR_PLAIN_WB, MDISCR_PLAIN_MINW, Discr>;
+    constexpr auto kern =
+        k_partition_lsd<NW, R, NBITS, MDISCR_PLAIN_WB, MDISCR_PLAIN_MINW, Discr, G::CLS>;
     static const size_t pad = lds_pad_for(kern, MDISCR_LSD_WG_PER_CU);
     TimedLaunch tl;
     timing_begin(ctx, s, tl);
@@ -506,7 +530,7 @@ mdiscr_status launch_plain(mdiscr_ctx* ctx, const uint32_t* in, uint32_t* out, u
                            Discr dis, uint32_t m, uint32_t* user_counts, uint64_t* offsets,
                            hipStream_t s, uint32_t* ws) {
     if constexpr (use_lsd<NBITS, Discr>()) {
-        if (((uintptr_t)in & 15u) == 0 && n > kLsdTile)
+        if (((uintptr_t)in & 15u) == 0 && n > LsdGeom<Discr>::TILE)
             return launch_plain_lsd<NBITS>(ctx, in, out, n, dis, m, user_counts, offsets, s, ws);
     }
     if constexpr (NBITS <= 4 && MDISCR_PLAIN_QUAD != 0) {

@@ -740,6 +740,46 @@ __device__ __forceinline__ void quad_rank(const u32x4 (&v)[Q], uint32_t (&ck)[Q]
     }
 }
 
+// quad_rank for digits given as bytes j of dq[q] (digit < 2^NBITS of element
+// j of quad q): the ranks within their (wave, quad, digit) group to rk, the
+// group sizes to qb[q][d].
+template <int Q, int NBITS>
+__device__ __forceinline__ void quad_rank_digits(const uint32_t (&dq)[Q], uint32_t (&rk)[Q],
+                                                 uint32_t (*qb)[1 << NBITS], uint32_t lane) {
+    using QR = QuadRank<NBITS>;
+    constexpr int MC = 1 << NBITS;
+    constexpr int K = QR::kScan;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        uint64_t h[2] = {0ull, 0ull};
+        uint32_t c[4], lp[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            c[j] = (dq[q] >> (8 * j)) & 0xFFu;
+            lp[j] = j ? QR::byte_of(h, c[j]) : 0u;
+            QR::add(h, c[j]);
+        }
+        uint32_t sc[K], own[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) own[k] = sc[k] = QR::half(h, k);
+        wave_incl_scan_dpp_n<K>(sc);
+        uint64_t ex[2] = {0ull, 0ull}, e63[2] = {0ull, 0ull}, h63[2] = {0ull, 0ull};
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const uint32_t e = sc[k] - own[k];
+            QR::set_half(ex, k, e);
+            QR::set_half(e63, k, (uint32_t)__builtin_amdgcn_readlane((int)e, kWave - 1));
+            QR::set_half(h63, k, (uint32_t)__builtin_amdgcn_readlane((int)own[k], kWave - 1));
+        }
+        uint32_t rw = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) rw |= (QR::byte_of(ex, c[j]) + lp[j]) << (8 * j);
+        asm volatile("" : "+v"(rw));
+        rk[q] = rw;
+        if (lane < (uint32_t)MC) qb[q][lane] = QR::byte_of(e63, lane) + QR::byte_of(h63, lane);
+    }
+}
+
 template <int NW, int R, int NBITS, int WB, bool OUT16, int MINW, class Discr, bool SINGLE = false>
 __global__ __launch_bounds__(NW* kWave, MINW) void k_partition_quad(
     const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t n, Discr dis,
@@ -953,7 +993,7 @@ __device__ __forceinline__ void stage_quads(uint32_t* stage, const u32x4 (&v)[Q]
 // n are given the value m - 1, whose class is m - 1 for the mask and the
 // modulus alike: both passes then rank them after every real key of class
 // m - 1, in the LDS slots at or past `valid`, which are never written out.
-template <int NW, int R, int NBITS, int WB, int MINW, class Discr>
+template <int NW, int R, int NBITS, int WB, int MINW, class Discr, bool CLS = false>
 __global__ __launch_bounds__(NW* kWave, MINW) void k_partition_lsd(
     const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t n, Discr dis,
     const uint32_t* __restrict__ prefix, const uint32_t* __restrict__ totals, uint32_t ld,
@@ -970,6 +1010,9 @@ __global__ __launch_bounds__(NW* kWave, MINW) void k_partition_lsd(
     __shared__ uint32_t qlo[NW][Q][LC];  // group sizes, then slot bases: low digit
     __shared__ uint32_t qhi[NW][Q][HC];  // the same for the high digit
     __shared__ uint32_t dpos[1 << NBITS];  // output index - LDS slot, per class
+    // CLS: each slot's class as a byte, carried from pass 1 to pass 2 and to
+    // the write-out, so that the class is evaluated once per key.
+    __shared__ __attribute__((aligned(16))) uint8_t cls[CLS ? TILE : 4];
 
     const uint32_t lane = lane_id();
     const uint32_t wave = wave_id();
@@ -1007,8 +1050,18 @@ __global__ __launch_bounds__(NW* kWave, MINW) void k_partition_lsd(
 
     // Pass 1: rank by the low digit, stage in (digit, memory) order.
     uint32_t ck[Q], rk[Q];
-    quad_rank<Q, LB, true>(v, ck, rk, qlo[wave], 0u, 0u, m, lane,
-                           ClassDigit<Discr, 0, LC - 1>{dis});
+    [[maybe_unused]] uint32_t cp[CLS ? Q : 1];  // CLS: the classes, bytes j of cp[q]
+    if constexpr (CLS) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            cp[q] = dis(v[q].x) | dis(v[q].y) << 8 | dis(v[q].z) << 16 | dis(v[q].w) << 24;
+            ck[q] = cp[q] & 0x07070707u;
+        }
+        quad_rank_digits<Q, LB>(ck, rk, qlo[wave], lane);
+    } else {
+        quad_rank<Q, LB, true>(v, ck, rk, qlo[wave], 0u, 0u, m, lane,
+                               ClassDigit<Discr, 0, LC - 1>{dis});
+    }
     __syncthreads();
     if (wave == 0) {
         digit_bases<NW, Q, LC>(qlo, lane);
@@ -1021,23 +1074,62 @@ __global__ __launch_bounds__(NW* kWave, MINW) void k_partition_lsd(
         }
     }
     __syncthreads();
-    stage_quads<Q, LC>(stage, v, ck, rk, qlo[wave]);
+    if constexpr (CLS) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const uint32_t x[4] = {v[q].x, v[q].y, v[q].z, v[q].w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t sl = qlo[wave][q][(ck[q] >> (8 * j)) & 0xFFu] + ((rk[q] >> (8 * j)) & 0xFFu);
+                stage[sl] = x[j];
+                cls[sl] = (uint8_t)(cp[q] >> (8 * j));
+            }
+        }
+    } else {
+        stage_quads<Q, LC>(stage, v, ck, rk, qlo[wave]);
+    }
     __syncthreads();
 
     // Pass 2: the wave's share of that order, ranked by the high digit.
     const u32x4* sp = reinterpret_cast<const u32x4*>(stage + wave * WT) + lane;
 #pragma unroll
     for (int q = 0; q < Q; ++q) v[q] = sp[q * kWave];
-    quad_rank<Q, HB, true>(v, ck, rk, qhi[wave], 0u, 0u, m, lane,
-                           ClassDigit<Discr, LB, HC - 1>{dis});
-    __syncthreads();  // every wave has read its share of stage
+    if constexpr (CLS) {
+        // The classes of the same four keys: one 32-bit LDS read per quad.
+        const uint32_t* cw = reinterpret_cast<const uint32_t*>(cls + wave * WT) + lane;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            cp[q] = cw[q * kWave];
+            ck[q] = (cp[q] >> LB) & (0x01010101u * (HC - 1u));
+        }
+        quad_rank_digits<Q, HB>(ck, rk, qhi[wave], lane);
+    } else {
+        quad_rank<Q, HB, true>(v, ck, rk, qhi[wave], 0u, 0u, m, lane,
+                               ClassDigit<Discr, LB, HC - 1>{dis});
+    }
+    __syncthreads();  // every wave has read its share of stage (and cls)
     if (wave == 0) digit_bases<NW, Q, HC>(qhi, lane);
     __syncthreads();
-    stage_quads<Q, HC>(stage, v, ck, rk, qhi[wave]);
-    __syncthreads();
-
-    if (full) write_runs<R, THREADS, WB, true, MDISCR_QUAD_NT_STORE>(stage, dpos, out, valid, dis);
-    else write_runs<R, THREADS, WB, false, MDISCR_QUAD_NT_STORE>(stage, dpos, out, valid, dis);
+    if constexpr (CLS) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const uint32_t x[4] = {v[q].x, v[q].y, v[q].z, v[q].w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t sl = qhi[wave][q][(ck[q] >> (8 * j)) & 0xFFu] + ((rk[q] >> (8 * j)) & 0xFFu);
+                stage[sl] = x[j];
+                cls[sl] = (uint8_t)(cp[q] >> (8 * j));
+            }
+        }
+        __syncthreads();
+        if (full) write_runs_cls<R, THREADS, WB, true, MDISCR_QUAD_NT_STORE>(stage, cls, dpos, out, valid);
+        else write_runs_cls<R, THREADS, WB, false, MDISCR_QUAD_NT_STORE>(stage, cls, dpos, out, valid);
+    } else {
+        stage_quads<Q, HC>(stage, v, ck, rk, qhi[wave]);
+        __syncthreads();
+        if (full) write_runs<R, THREADS, WB, true, MDISCR_QUAD_NT_STORE>(stage, dpos, out, valid, dis);
+        else write_runs<R, THREADS, WB, false, MDISCR_QUAD_NT_STORE>(stage, dpos, out, valid, dis);
+    }
 }
 
 // d_counts[c] = 0 for n == 0, and other small fills (a kernel rather than a
