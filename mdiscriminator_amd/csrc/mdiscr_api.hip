@@ -410,11 +410,11 @@ mdiscr_status launch_plain_quad(mdiscr_ctx* ctx, const uint32_t* in, uint32_t* o
 #define MDISCR_LSD_WG_PER_CU 0
 #endif
 
-// Where the two-digit path serves: 17..64 classes, and 16 classes under the
-// mask (2^28 keys: 0.412 -> 0.386 ms).  Under the modulus, at 9..16 classes,
-// the one-pass quad kernel stays: the two-digit kernel evaluates x % m once
-// per pass and once more for the write-out, which ate its gain (m = 9 / 13:
-// 0.463 / 0.463 ms one-pass against 0.466 / 0.470 ms).
+// Where the two-digit path serves: 17..64 classes, 16 classes under the mask
+// (2^28 keys: 0.412 -> 0.386 ms), and 9..16 under the modulus when it carries
+// the classes (m = 9 / 13: 0.437 / 0.439 ms one-pass against 0.398 / 0.406;
+// evaluating x % m in every pass instead, it lost: 0.463 / 0.463 against
+// 0.466 / 0.470).
 #ifndef MDISCR_LSD_MOD16
 #define MDISCR_LSD_MOD16 1
 #endif
