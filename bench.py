@@ -140,20 +140,44 @@ def first_steps(ctx, step, stream, steps):
     print(json.dumps({"first_steps": rows}), flush=True)
 
 
-def run_driver(args, timeout):
-    """Runs a C host driver of the C ABI and returns its JSON line (or an error)."""
+def run_driver(args, timeout, tuned=False):
+    """Runs a C host driver of the C ABI and returns its JSON line (or an error).
+
+    tuned: the driver's autotuner writes its choices to a fresh cache file
+    (MDISCR_TUNE_CACHE), returned under "tuned" as {bucket: candidate name}."""
     import subprocess
+    import tempfile
     exe = os.path.join(ROOT, "mdiscriminator_amd", "bin", args[0])
     if not os.path.exists(exe):
         return {"error": f"{args[0]} not built"}
+    env = dict(os.environ)
+    tdir = tempfile.mkdtemp(prefix="mdiscr_tune_") if tuned else None
+    if tuned:
+        env["MDISCR_TUNE_CACHE"] = os.path.join(tdir, "tune.txt")
+        env.pop("MDISCR_SEG_CONFIG", None)
+        env.pop("MDISCR_AUTOTUNE", None)
     try:
         r = subprocess.run([exe] + [str(a) for a in args[1:]], capture_output=True, text=True,
-                           timeout=timeout)
+                           timeout=timeout, env=env)
     except subprocess.TimeoutExpired:
         return {"error": f"{args[0]} timed out after {timeout} s"}
     if r.returncode != 0:
         return {"error": f"{args[0]} exit {r.returncode}: {r.stderr[-300:]}"}
-    return json.loads(r.stdout.strip().splitlines()[-1])
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    if tuned:
+        choices = {}
+        path = env["MDISCR_TUNE_CACHE"]
+        if os.path.exists(path):
+            # lines "<bucket>=<candidate index> <candidate name>"
+            with open(path) as fh:
+                for ln in fh:
+                    bucket, _, rest = ln.strip().rpartition("=")
+                    if bucket:
+                        choices[bucket] = rest.split(" ", 1)[-1]
+            os.remove(path)
+        os.rmdir(tdir)
+        out["tuned"] = choices
+    return out
 
 
 def main():
@@ -368,6 +392,11 @@ def main():
         ty = (W.uniform_u31_torch(nt, SEED + 4, "cuda:0") & 0xFF).to(torch.uint8)
         tz = (W.uniform_u31_torch(nt, SEED + 5, "cuda:0") & 1).to(torch.uint8)
         touts = [torch.empty_like(t) for t in (tx, ty, tz)]
+        # The reference's own tuple test: (uint, uchar, bool) columns, class
+        # from the first (ModTuple4, MdiscrHost_tuples.cu.h:14-29).
+        general["ref_tuple_2p26_m4"] = shape_line(
+            lambda: ctx.partition_soa([tx, ty, tz], touts, md.Mask(4), 0, stream=stream),
+            nt, 2 * 6, ["soa_count", "soa_scan", "soa_partition"], "soa_partition")
         general["select_tuples_2p26_m8"] = shape_line(
             lambda: ctx.partition_soa_select([tx, ty, tz], touts, md.Mask(8), 2, 0, 1,
                                              stream=stream),
@@ -377,7 +406,7 @@ def main():
     seg = serve = None
     if not args.no_drivers:
         log("batched segmented vs one plain launch per segment (C driver)")
-        seg = run_driver(["mdiscr_segbench", 10000, 4096, 1.5, 4, 5], 300)
+        seg = run_driver(["mdiscr_segbench", 10000, 4096, 1.5, 4, 5], 300, tuned=True)
         log(f"serving {args.serve_requests} requests (C driver)")
         serve = run_driver(["mdiscr_servebench", args.serve_requests, 0.8, 64, 4, 16], 400)
 

@@ -9,6 +9,7 @@
 
 #include "api_internal.hpp"
 #include "soa_kernels.hpp"
+#include "soa_quad_kernels.hpp"
 
 using namespace mdiscr;
 
@@ -100,7 +101,7 @@ static_assert(SoaGeom<uint32_t, 5>::TILE == SoaGeom<uint32_t, 3>::TILE &&
 template <int NBITS, typename ST, class Src, class Discr>
 mdiscr_status launch_soa(mdiscr_ctx* ctx, const SoaCols& cols, Src src, uint32_t n, Discr dis,
                          uint32_t m, uint32_t* user_counts, hipStream_t s, uint32_t* ws) {
-    using G = SoaGeom<ST, NBITS, std::is_same_v<Src, SelectSrc>>;
+    using G = SoaGeom<ST, NBITS, IsSelect<Src>::value>;
     constexpr int NW = G::NW, R = G::R;
     constexpr int kCCT = G::CCT;
     constexpr uint32_t TILE = G::TILE;
@@ -146,9 +147,103 @@ mdiscr_status launch_soa(mdiscr_ctx* ctx, const SoaCols& cols, Src src, uint32_t
     return ctx_hip(ctx, hipGetLastError(), "SoA partition launch");
 }
 
+// The quad-layout partition (up to 16 classes, every column of 1, 2 or 4
+// bytes and aligned to 4 elements): NW waves x 64 lanes x Q quads per tile.
+// 2^26 records, partition ms against the ballot-ranked kernel (interleaved
+// A/B): (u32, u8, u8) keyed on the u32 at m=4 0.191 -> 0.165, m=16 0.207 ->
+// 0.185; select z ? x & 7 : y & 7 over (u32, u8, u8) 0.221 -> 0.149; a 1-byte
+// class array with a u32 payload, m=16 0.171 -> 0.149.  Tiles of 8 x 2, 4 x 2
+// and 4 x 3 quads per lane are within 3% of each other.
+#ifndef MDISCR_SOA_QUAD
+#define MDISCR_SOA_QUAD 1
+#endif
+#ifndef MDISCR_SOAQ_NW
+#define MDISCR_SOAQ_NW 4
+#endif
+#ifndef MDISCR_SOAQ_Q
+#define MDISCR_SOAQ_Q 2
+#endif
+#ifndef MDISCR_SOAQ_WG_PER_CU
+#define MDISCR_SOAQ_WG_PER_CU 0
+#endif
+constexpr uint32_t kQuadTile = MDISCR_SOAQ_NW * kWave * 4 * MDISCR_SOAQ_Q;
+
+#ifndef MDISCR_SOAQ_ALL
+#define MDISCR_SOAQ_ALL 0
+#endif
+// Whether the quad-layout partition takes these columns: it can when every
+// column holds 1-, 2- or 4-byte elements aligned to 4 elements, and it is
+// faster when some column is narrower than 4 bytes or the class comes from a
+// select.  For 4-byte columns alone the ballot-ranked kernel is 2-3% faster
+// (2^26 (key, f32, index) records, m=8 / 16).
+bool quad_columns(const SoaCols& cols, bool select) {
+    bool narrow = false;
+    for (uint32_t j = 0; j < cols.ncols; ++j) {
+        const uint32_t b = cols.bytes[j];
+        if (b > 4) return false;
+        if ((((uintptr_t)cols.in[j] | (uintptr_t)cols.out[j]) % (4u * b)) != 0) return false;
+        narrow |= b < 4;
+    }
+    return narrow || select || MDISCR_SOAQ_ALL;
+}
+
+template <int NBITS, class Src, class Discr>
+mdiscr_status launch_soa_quad(mdiscr_ctx* ctx, const SoaCols& cols, Src src, uint32_t n,
+                              Discr dis, uint32_t m, uint32_t* user_counts, hipStream_t s,
+                              uint32_t* ws) {
+    constexpr int NW = MDISCR_SOAQ_NW, Q = MDISCR_SOAQ_Q;
+    constexpr uint32_t TILE = kQuadTile;
+    const uint32_t T = (uint32_t)(((uint64_t)n + TILE - 1) / TILE);
+    const uint32_t ld = (T + 3u) & ~3u;
+    uint32_t* counts = ws;
+    uint32_t* prefix = ws + (size_t)m * ld;
+    uint32_t* totals = ws + (size_t)2 * m * ld;
+    TimedLaunch tl;
+    if (T == 1) {
+        timing_begin(ctx, s, tl);
+        hipLaunchKernelGGL((k_partition_soa_quad<NW, Q, NBITS, Src, Discr, true>), dim3(1),
+                           dim3(NW * kWave), 0, s, cols, src, n, dis, nullptr, nullptr, 0u, m,
+                           user_counts);
+        timing_end(ctx, s, tl, "soa_partition");
+        return ctx_hip(ctx, hipGetLastError(), "SoA partition launch");
+    }
+    timing_begin(ctx, s, tl);
+    if constexpr (std::is_same_v<Src, KeySrc<uint32_t>>) {
+        // 16-byte aligned keys (quad_columns): 16-byte count loads.
+        hipLaunchKernelGGL((k_chunk_count<NW, 4 * Q, 1, NBITS, true, Discr>), dim3(T),
+                           dim3(NW * kWave), 0, s, src.key, n, dis, counts, ld, m);
+    } else {
+        hipLaunchKernelGGL((k_src_count<NW, 4 * Q, 1, NBITS, Src, Discr>), dim3(T),
+                           dim3(NW * kWave), 0, s, src, n, dis, counts, ld, m);
+    }
+    timing_end(ctx, s, tl, "soa_count");
+    timing_begin(ctx, s, tl);
+    launch_tile_scan(counts, prefix, T, ld, m, totals, user_counts, nullptr, s);
+    timing_end(ctx, s, tl, "soa_scan");
+    constexpr auto kern = k_partition_soa_quad<NW, Q, NBITS, Src, Discr>;
+    static const size_t pad = lds_pad_for(kern, MDISCR_SOAQ_WG_PER_CU);
+    timing_begin(ctx, s, tl);
+    hipLaunchKernelGGL(kern, dim3(T), dim3(NW * kWave), pad, s, cols, src, n, dis, prefix, totals,
+                       ld, m, nullptr);
+    timing_end(ctx, s, tl, "soa_partition");
+    return ctx_hip(ctx, hipGetLastError(), "SoA partition launch");
+}
+
 template <typename ST, class Src, class Discr>
 mdiscr_status dispatch_bits(mdiscr_ctx* ctx, const SoaCols& cols, Src src, uint32_t n, Discr dis,
                             uint32_t m, uint32_t* user_counts, hipStream_t s, uint32_t* ws) {
+    if constexpr (sizeof(ST) == 4 && MDISCR_SOA_QUAD) {
+        if (quad_columns(cols, IsSelect<Src>::value)) {
+            switch (class_bits(m)) {
+                case 0: return launch_soa_quad<0>(ctx, cols, src, n, dis, m, user_counts, s, ws);
+                case 1: return launch_soa_quad<1>(ctx, cols, src, n, dis, m, user_counts, s, ws);
+                case 2: return launch_soa_quad<2>(ctx, cols, src, n, dis, m, user_counts, s, ws);
+                case 3: return launch_soa_quad<3>(ctx, cols, src, n, dis, m, user_counts, s, ws);
+                case 4: return launch_soa_quad<4>(ctx, cols, src, n, dis, m, user_counts, s, ws);
+                default: break;
+            }
+        }
+    }
     switch (class_bits(m)) {
         case 0: return launch_soa<0, ST>(ctx, cols, src, n, dis, m, user_counts, s, ws);
         case 1: return launch_soa<1, ST>(ctx, cols, src, n, dis, m, user_counts, s, ws);
@@ -192,11 +287,15 @@ mdiscr_status run_staged(mdiscr_ctx* ctx, const SoaCols& cols, uint32_t n, const
             return dispatch_discr<ST>(
                 ctx, cols, KeySrc<uint32_t>{static_cast<const uint32_t*>(cols.in[cols.key]), 1u},
                 n, cs.d, d_counts, s, w);
-        case ClassKind::Pred:
-            return launch_soa<1, ST>(
-                ctx, cols,
-                KeySrc<uint32_t>{static_cast<const uint32_t*>(cols.in[cols.key]), 1u},
-                n, DiscrPred{cs.pw[0], cs.pw[1], cs.pw[2], cs.pw[3]}, 2u, d_counts, s, w);
+        case ClassKind::Pred: {
+            const KeySrc<uint32_t> ks{static_cast<const uint32_t*>(cols.in[cols.key]), 1u};
+            const DiscrPred dp{cs.pw[0], cs.pw[1], cs.pw[2], cs.pw[3]};
+            if constexpr (sizeof(ST) == 4 && MDISCR_SOA_QUAD) {
+                if (quad_columns(cols, false))
+                    return launch_soa_quad<1>(ctx, cols, ks, n, dp, 2u, d_counts, s, w);
+            }
+            return launch_soa<1, ST>(ctx, cols, ks, n, dp, 2u, d_counts, s, w);
+        }
         case ClassKind::Select:
             return dispatch_discr<ST>(
                 ctx, cols,
@@ -224,7 +323,9 @@ mdiscr_status run_soa(mdiscr_ctx* ctx, SoaCols& cols, uint64_t n, const ClassSpe
     }
     bool wide = false;
     for (uint32_t j = 0; j < cols.ncols; ++j) wide |= cols.bytes[j] > 4;
-    const uint64_t tile = soa_tile(wide, class_bits(m), cs.kind == ClassKind::Select);
+    // sized by the smallest tile the call may use
+    const uint64_t tile = std::min<uint64_t>(
+        soa_tile(wide, class_bits(m), cs.kind == ClassKind::Select), kQuadTile);
     const uint64_t T = (n + tile - 1) / tile;
     const uint64_t ld = (T + 3) & ~3ull;
     void* ws = nullptr;

@@ -131,6 +131,11 @@ struct SelectSrc {
     }
 };
 
+template <class Src>
+struct IsSelect : std::false_type {};
+template <>
+struct IsSelect<SelectSrc> : std::true_type {};
+
 // The select source's partition writes its three columns from the values it
 // loaded for the classes instead of loading them again.
 #ifndef MDISCR_SEL_KEEP
