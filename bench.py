@@ -256,6 +256,18 @@ def main():
                     ctx.partition_soa(cols, outs, md.Mask(m_), stream=stream)
             torch.cuda.synchronize()
             del cols, outs
+            # (uint, uchar, bool) records: keyed on the uint (x & 3), then the
+            # select z ? x & 7 : y & 7 (the quad-layout kernel).
+            cols = [W.uniform_u31_torch(ns, SEED + 3, "cuda:0"),
+                    (W.uniform_u31_torch(ns, SEED + 4, "cuda:0") & 0xFF).to(torch.uint8),
+                    (W.uniform_u31_torch(ns, SEED + 5, "cuda:0") & 1).to(torch.uint8)]
+            outs = [torch.empty_like(c) for c in cols]
+            for _ in range(args.steps):
+                ctx.partition_soa(cols, outs, md.Mask(4), 0, stream=stream)
+            for _ in range(args.steps):
+                ctx.partition_soa_select(cols, outs, md.Mask(8), 2, 0, 1, stream=stream)
+            torch.cuda.synchronize()
+            del cols, outs
         f_in, f_out = d_in.view(torch.float32), d_out.view(torch.float32)
         for _ in range(args.steps):
             torch.neg(f_in, out=f_out)   # 1 GiB read + 1 GiB write, counter calibration

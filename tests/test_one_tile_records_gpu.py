@@ -3,8 +3,10 @@ TILE+1) on the GPU, against the oracle: at most one tile takes the single
 launch that counts its own tile, one more takes the full count, scan and
 partition.  Class counts start filled with -1, so a count the call did not
 write shows up.  Tiles: SoA 4096 (m <= 32, 8-byte columns), 6144 (64
-classes); the select source 3072 (any class count); array
-rows 2048 / 2560 / 1536 / 1024 elements for LEN 2 / 3 / 5 / 8."""
+classes); records with a column narrower than 4 bytes, or a select source, up
+to 16 classes 2048 (the quad-layout kernel); the select source above 16
+classes 3072; array rows 2048 / 2560 / 1536 / 1024 elements for LEN
+2 / 3 / 5 / 8."""
 import numpy as np
 import pytest
 
@@ -29,6 +31,15 @@ def host(t, dtype):
 def counts_tensor(m):
     import torch
     return torch.full((m,), -1, dtype=torch.int32, device="cuda:0")
+
+
+def filled(c):
+    """An output column like c, every byte 0xEE (a value the call must overwrite)."""
+    import torch
+    return torch.full(c.shape, 0xEE, dtype=torch.uint8, device=c.device).view(c.dtype) \
+        if c.dtype == torch.uint8 else \
+        torch.full((c.numel() * c.element_size(),), 0xEE, dtype=torch.uint8,
+                   device=c.device).view(c.dtype)
 
 
 def edges(tile):
@@ -74,7 +85,7 @@ def test_soa_wide_column_one_tile(ctx):
         check_cols(outs, ref_outs, cnt, ref_cnt, f"wide n={n}")
 
 
-@pytest.mark.parametrize("m,tile", [(2, 3072), (8, 3072), (64, 3072)])
+@pytest.mark.parametrize("m,tile", [(2, 2048), (8, 2048), (16, 2048), (64, 3072)])
 def test_select_one_tile(ctx, m, tile):
     """(uint x, uchar y, bool z) records under z ? x % m : y % m."""
     import torch
@@ -92,6 +103,63 @@ def test_select_one_tile(ctx, m, tile):
         torch.cuda.synchronize()
         ref_outs, ref_cnt = oracle.seq_mdiscr_soa_select(cols, m, 2, 0, 1, oracle.MASK)
         check_cols(outs, ref_outs, cnt, ref_cnt, f"select m={m} n={n}")
+
+
+@pytest.mark.parametrize("shape", ["tuple_m4", "tuple_m16", "bytekey_mod7", "u16_m8",
+                                   "filter_bytes"])
+def test_quad_records_one_tile(ctx, shape):
+    """Records with a column narrower than 4 bytes, up to 16 classes (the
+    quad-layout kernel, 2048-record tiles)."""
+    import torch
+    import mdiscriminator_amd as md
+    for n in edges(2048):
+        x = W.uniform_u31(n, 1000 + n)
+        b = (W.uniform_u31(n, 1100 + n) & 0xFF).astype(np.uint8)
+        if shape.startswith("tuple"):
+            m = 4 if shape == "tuple_m4" else 16
+            cols, key, d, kind = [x, b, (x & 1).astype(np.uint8)], 0, md.Mask(m), oracle.MASK
+        elif shape == "bytekey_mod7":
+            m = 7
+            cols, key, d, kind = [(x % 7).astype(np.uint8), x], 0, md.Mod(7), oracle.MOD
+        elif shape == "u16_m8":
+            m = 8
+            cols = [(x & 0xFFFF).astype(np.uint16), x, b]
+            key, d, kind = 1, md.Mask(8), oracle.MASK
+        else:
+            m = 2
+            cols, key = [x, b], 0
+        d_cols = [dev(c) for c in cols]
+        outs = [filled(c) for c in d_cols]
+        cnt = counts_tensor(m)
+        print(f"quad records {shape} n={n}", flush=True)
+        if shape == "filter_bytes":
+            ctx.filter_soa(d_cols, outs, md.Pred.lt(1 << 30), d_counts=cnt)
+            torch.cuda.synchronize()
+            ref_outs, ref_cnt = oracle.filter_soa(cols, _lib.PRED_LT, 1 << 30)
+        else:
+            ctx.partition_soa(d_cols, outs, d, key, d_counts=cnt)
+            torch.cuda.synchronize()
+            ref_outs, ref_cnt = oracle.seq_mdiscr_soa(cols, m, key, kind)
+        check_cols(outs, ref_outs, cnt, ref_cnt, f"{shape} n={n}")
+
+
+@pytest.mark.parametrize("offset", [1, 2, 3])
+def test_quad_records_misaligned(ctx, offset):
+    """Byte columns that start off a 4-byte boundary take the ballot-ranked
+    kernel; the result is the same."""
+    import torch
+    import mdiscriminator_amd as md
+    n = 3 * 2048 + 5
+    x = W.uniform_u31(n, 1200 + offset)
+    b = (W.uniform_u31(n + offset, 1300 + offset) & 0xFF).astype(np.uint8)
+    d_b = dev(b)[offset:]
+    d_ob = torch.full((n + offset,), 0xEE, dtype=torch.uint8, device="cuda:0")[offset:]
+    d_x, d_ox = dev(x), torch.full((n,), -1, dtype=torch.int32, device="cuda:0")
+    cnt = counts_tensor(16)
+    ctx.partition_soa([d_x, d_b], [d_ox, d_ob], md.Mask(16), 0, d_counts=cnt)
+    torch.cuda.synchronize()
+    ref_outs, ref_cnt = oracle.seq_mdiscr_soa([x, b[offset:]], 16, 0, oracle.MASK)
+    check_cols([d_ox, d_ob], ref_outs, cnt, ref_cnt, f"misaligned by {offset}")
 
 
 def test_filter_records_one_tile(ctx):
