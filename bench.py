@@ -332,6 +332,27 @@ def main():
                                                     / 1e9 / PEAK_HBM_GBS, 4),
                             "kernels_ms": {k: round(v, 5) for k, v in ktm.items() if v}}
 
+    # The reference's own sweep: 50,000,000 keys, class = x % m (Mod<m>) for
+    # m = 2, 4, ..., 16 (cuda_code/run.py:6,66-82 with
+    # MdiscrTest_optimized_clean.cu:17-18); uniform keys in [0, 2^31).
+    ref_sweep = {}
+    if not args.no_drivers:
+        n_ref = 50_000_000
+        r_in, r_out = d_in[:n_ref], d_out[:n_ref]
+        for m_ in range(2, 17, 2):
+            dis = md.Mod(m_)
+
+            def ref_step():
+                ctx.partition_u32(r_in, r_out, dis, stream=stream)
+            dt_r = time_steps(ctx, ref_step, 20, 3)
+            ktr = kernel_times(ctx, ref_step, 20, kernels)
+            ms_r = dt_r / 20 * 1e3
+            ref_sweep[f"mod{m_}"] = {
+                "ms_per_call": round(ms_r, 4), "gelem_s": round(n_ref / ms_r / 1e6, 2),
+                "partition_frac": round(n_ref * BYTES_PER_ELEM / (ktr["partition"] * 1e-3)
+                                        / 1e9 / PEAK_HBM_GBS, 4),
+                "kernels_ms": {k: round(v, 5) for k, v in ktr.items() if v}}
+
     # Calls of at most one tile (one launch each), stream-ordered back to back.
     small = {}
     if not args.no_drivers:
@@ -463,6 +484,7 @@ def main():
         "zipf_variant": {"value": round(N * args.steps / dtz / 1e9, 3), "unit": "Gelem/s",
                          "kernels_ms": {k: round(v, 5) for k, v in ktz.items() if v}},
         "plain_across_m": sweep or None,
+        "reference_sweep_5e7": ref_sweep or None,
         "small_calls": small or None,
         "soa_records_2p26": soa or None,
         "general": general or None,
