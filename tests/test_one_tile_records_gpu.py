@@ -105,7 +105,7 @@ def test_select_one_tile(ctx, m, tile):
         check_cols(outs, ref_outs, cnt, ref_cnt, f"select m={m} n={n}")
 
 
-@pytest.mark.parametrize("shape", ["tuple_m4", "tuple_m16", "bytekey_mod7", "u16_m8",
+@pytest.mark.parametrize("shape", ["tuple_m1", "tuple_m4", "tuple_m16", "bytekey_mod7", "u16_m8",
                                    "filter_bytes"])
 def test_quad_records_one_tile(ctx, shape):
     """Records with a column narrower than 4 bytes, up to 16 classes (the
@@ -116,7 +116,7 @@ def test_quad_records_one_tile(ctx, shape):
         x = W.uniform_u31(n, 1000 + n)
         b = (W.uniform_u31(n, 1100 + n) & 0xFF).astype(np.uint8)
         if shape.startswith("tuple"):
-            m = 4 if shape == "tuple_m4" else 16
+            m = int(shape[len("tuple_m"):])
             cols, key, d, kind = [x, b, (x & 1).astype(np.uint8)], 0, md.Mask(m), oracle.MASK
         elif shape == "bytekey_mod7":
             m = 7
@@ -143,23 +143,25 @@ def test_quad_records_one_tile(ctx, shape):
         check_cols(outs, ref_outs, cnt, ref_cnt, f"{shape} n={n}")
 
 
-@pytest.mark.parametrize("offset", [1, 2, 3])
-def test_quad_records_misaligned(ctx, offset):
-    """Byte columns that start off a 4-byte boundary take the ballot-ranked
-    kernel; the result is the same."""
+@pytest.mark.parametrize("offset,key_offset", [(1, 0), (2, 0), (3, 0), (0, 1), (0, 2)])
+def test_quad_records_misaligned(ctx, offset, key_offset):
+    """Columns that start off a 4-element boundary (a byte column `offset`
+    bytes, the u32 key column `key_offset` words past one) take the
+    ballot-ranked kernel; the result is the same."""
     import torch
     import mdiscriminator_amd as md
     n = 3 * 2048 + 5
-    x = W.uniform_u31(n, 1200 + offset)
+    x = W.uniform_u31(n + key_offset, 1200 + offset + 7 * key_offset)
     b = (W.uniform_u31(n + offset, 1300 + offset) & 0xFF).astype(np.uint8)
     d_b = dev(b)[offset:]
     d_ob = torch.full((n + offset,), 0xEE, dtype=torch.uint8, device="cuda:0")[offset:]
-    d_x, d_ox = dev(x), torch.full((n,), -1, dtype=torch.int32, device="cuda:0")
+    d_x = dev(x)[key_offset:]
+    d_ox = torch.full((n + key_offset,), -1, dtype=torch.int32, device="cuda:0")[key_offset:]
     cnt = counts_tensor(16)
     ctx.partition_soa([d_x, d_b], [d_ox, d_ob], md.Mask(16), 0, d_counts=cnt)
     torch.cuda.synchronize()
-    ref_outs, ref_cnt = oracle.seq_mdiscr_soa([x, b[offset:]], 16, 0, oracle.MASK)
-    check_cols([d_ox, d_ob], ref_outs, cnt, ref_cnt, f"misaligned by {offset}")
+    ref_outs, ref_cnt = oracle.seq_mdiscr_soa([x[key_offset:], b[offset:]], 16, 0, oracle.MASK)
+    check_cols([d_ox, d_ob], ref_outs, cnt, ref_cnt, f"misaligned by {offset} / {key_offset}")
 
 
 def test_filter_records_one_tile(ctx):
