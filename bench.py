@@ -353,6 +353,24 @@ def main():
                                         / 1e9 / PEAK_HBM_GBS, 4),
                 "kernels_ms": {k: round(v, 5) for k, v in ktr.items() if v}}
 
+    # The CPU configuration on the GPU: 2^20 uniform keys, m = 4 (Mod4), the
+    # values SeqMdiscr is timed on in cpu_baseline's config 1, stream-ordered
+    # back to back (the call is launch-bound at this size).
+    config1 = {}
+    if not args.no_drivers:
+        n1 = 1 << 20
+        x1 = W.uniform_u31_torch(n1, W.SEED_BASE + 1, "cuda:0")
+        o1 = torch.empty_like(x1)
+        c1 = torch.empty(4, dtype=torch.int32, device="cuda:0")
+
+        def c1_step():
+            ctx.partition_u32(x1, o1, md.Mod4, d_counts=c1, stream=stream)
+        reps = 500
+        dt_c1 = time_steps(ctx, c1_step, reps, 20)
+        config1 = {"n": n1, "m": 4, "us_per_call": round(dt_c1 / reps * 1e6, 2),
+                   "gelem_s": round(n1 * reps / dt_c1 / 1e9, 2), "calls": reps}
+        del x1, o1
+
     # Calls of at most one tile (one launch each), stream-ordered back to back.
     small = {}
     if not args.no_drivers:
@@ -485,6 +503,7 @@ def main():
                          "kernels_ms": {k: round(v, 5) for k, v in ktz.items() if v}},
         "plain_across_m": sweep or None,
         "reference_sweep_5e7": ref_sweep or None,
+        "config1_gpu": config1 or None,
         "small_calls": small or None,
         "soa_records_2p26": soa or None,
         "general": general or None,
