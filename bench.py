@@ -210,8 +210,8 @@ def main():
                          "(key, f32, index) records at m = 16, 32, 64")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              f"(the partition shards by an all-to-all of class runs, not measured here)",
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (across GPUs "
+              f"the partition would shard by an all-to-all of class runs, not measured here)",
               file=sys.stderr)
         sys.exit(2)
 

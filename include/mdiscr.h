@@ -46,7 +46,7 @@ typedef enum {
     MDISCR_ENOMEM = 2,      /* device or host allocation failed / over budget */
     MDISCR_EDEVICE = 3,     /* a HIP runtime call failed */
     MDISCR_EBUSY = 4,       /* serving queue full */
-    MDISCR_EUNSUPPORTED = 5 /* valid request outside what this build handles */
+    MDISCR_EUNSUPPORTED = 5 /* valid request outside what the library handles */
 } mdiscr_status;
 
 /* Discriminator kinds: the class of a 32-bit key x. */
