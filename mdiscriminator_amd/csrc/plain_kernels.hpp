@@ -62,6 +62,23 @@
 #ifndef MDISCR_COUNT_NT_LOAD
 #define MDISCR_COUNT_NT_LOAD 1
 #endif
+
+// Copies of each wave's class histogram in the atomic count: lane l adds into
+// copy l % REP, so the lanes of one class spread over REP words instead of
+// serialising on one (2^28 keys, x % 9: count 0.194 -> 0.150 ms with 4
+// copies; x % 13: 0.165 -> 0.151).
+#ifndef MDISCR_COUNT_REP
+#define MDISCR_COUNT_REP 4
+#endif
+// Under x % m the count takes the atomic histogram at every class count (1),
+// or packed byte counters up to 8 classes as under the mask (0).  The packed
+// add (a 64-bit shift and add per key) on top of the modulus's two
+// quarter-rate multiplies costs more VALU time than one LDS atomic: 2^28
+// keys, count 0.173 -> 0.163 ms at x % 3 / 5 / 6 / 7 (0.160 under the mask);
+// 2^26 SoA keys, x % 7: 0.049 -> 0.044 ms.
+#ifndef MDISCR_MOD_ATOMIC_COUNT
+#define MDISCR_MOD_ATOMIC_COUNT 1
+#endif
 // Partition tiles in reverse order within each XCD's range: the count pass
 // reads every range front to back, so its last tiles may still sit in the
 // memory-side cache when the partition starts on them.
@@ -144,6 +161,23 @@ struct LaneCounts {
     }
 };
 
+template <int NBITS, class Discr>
+constexpr bool count_packed() {
+    return LaneCounts<NBITS>::kPacked && !(MDISCR_MOD_ATOMIC_COUNT && CostlyDiscr<Discr>::value);
+}
+
+// The atomic count's per-wave histogram: kRep copies of each class's word,
+// lane l adding into copy l % kRep, at most 64 words.
+template <int MC>
+struct HistRep {
+    static constexpr int kRep = (MC * MDISCR_COUNT_REP <= 64) ? MDISCR_COUNT_REP
+                                : (64 / MC > 1 ? 64 / MC : 1);
+    static constexpr int kWords = MC * kRep;
+    __device__ __forceinline__ static uint32_t at(uint32_t c, uint32_t lane) {
+        return c * kRep + (lane & (kRep - 1));
+    }
+};
+
 // ---------------------------------------------------------------------------
 // count pass: each workgroup reads CT consecutive tiles and writes every
 // tile's class counts, counts[c*ld + t] (class-major rows of ld >= T entries).
@@ -157,10 +191,11 @@ __global__ __launch_bounds__(NW* kWave) void k_chunk_count(
     constexpr int MC = 1 << NBITS;
     constexpr int TILE = G::kTile;
     constexpr int THREADS = G::kThreads;
-    constexpr bool PACKED = LaneCounts<NBITS>::kPacked;
+    constexpr bool PACKED = count_packed<NBITS, Discr>();
     static_assert(R <= 255, "byte counters per lane");
     static_assert(R % 4 == 0, "16-byte loads");
-    __shared__ uint32_t hist[2][NW][MC];  // double-buffered across tiles
+    using HR = HistRep<MC>;
+    __shared__ uint32_t hist[2][NW][PACKED ? MC : HR::kWords];  // double-buffered across tiles
 
     const uint32_t lane = lane_id();
     const uint32_t wave = wave_id();
@@ -187,9 +222,9 @@ __global__ __launch_bounds__(NW* kWave) void k_chunk_count(
     u32x4 q[R / 4];
     if (VEC && MDISCR_COUNT_PREFETCH && t0 < t1 && tile_full(t0)) vec_load(t0, q);
     for (uint32_t t = t0; t < t1; ++t) {
-        uint32_t (*h)[MC] = hist[(t - t0) & 1u];
+        uint32_t (*h)[PACKED ? MC : HR::kWords] = hist[(t - t0) & 1u];
         if (!PACKED) {
-            for (int c = lane; c < MC; c += kWave) h[wave][c] = 0;
+            for (int c = lane; c < HR::kWords; c += kWave) h[wave][c] = 0;
             __builtin_amdgcn_wave_barrier();
         }
         LaneCounts<NBITS> lc;
@@ -207,7 +242,7 @@ __global__ __launch_bounds__(NW* kWave) void k_chunk_count(
                 for (int j = 0; j < 4; ++j) {
                     const uint32_t c = dis(e[j]);
                     if (PACKED) lc.add(c);
-                    else atomicAdd(&h[wave][c], 1u);
+                    else atomicAdd(&h[wave][HR::at(c, lane)], 1u);
                 }
             }
             if (pre) {
@@ -227,7 +262,7 @@ __global__ __launch_bounds__(NW* kWave) void k_chunk_count(
                 if (full || idx < n) {
                     const uint32_t c = dis(v[k]);
                     if (PACKED) lc.add(c);
-                    else atomicAdd(&h[wave][c], 1u);
+                    else atomicAdd(&h[wave][HR::at(c, lane)], 1u);
                 }
             }
         }
@@ -239,7 +274,14 @@ __global__ __launch_bounds__(NW* kWave) void k_chunk_count(
         for (uint32_t c = threadIdx.x; c < m; c += THREADS) {
             uint32_t s = 0;
 #pragma unroll
-            for (int w = 0; w < NW; ++w) s += h[w][c];
+            for (int w = 0; w < NW; ++w) {
+                if (PACKED) {
+                    s += h[w][c];
+                } else {
+#pragma unroll
+                    for (int k = 0; k < HR::kRep; ++k) s += h[w][c * HR::kRep + k];
+                }
+            }
             counts[(size_t)c * ld + t] = s;
         }
     }

@@ -21,6 +21,8 @@
 // in a graph once and replayed with new contents.
 #pragma once
 
+#include <type_traits>
+
 #include "plain_kernels.hpp"
 
 // Output stores of the segmented partition (1 = non-temporal hint).  Unlike
@@ -328,9 +330,11 @@ __global__ __launch_bounds__(NW* kWave) void k_seg_count(const SegDesc* __restri
     constexpr int MC = 1 << NBITS;
     constexpr int THREADS = NW * kWave;
     constexpr uint32_t TILE = NW * kWave * R;
-    constexpr bool PACKED = LaneCounts<NBITS>::kPacked;
+    constexpr bool PACKED = count_packed<NBITS, std::conditional_t<MASK, DiscrMask, DiscrMod>>();
+    using HR = HistRep<MC>;
+    constexpr int HW = PACKED ? MC : HR::kWords;  // histogram words per wave
     constexpr uint32_t P = R <= 8 ? MDISCR_SEG_COUNT_P : 1u;  // items per iteration
-    __shared__ uint32_t hist[P][NW][MC];
+    __shared__ uint32_t hist[P][NW][HW];
     const uint32_t lane = lane_id();
     const uint32_t wave = wave_id();
     const uint32_t n_items = plan->n_items;
@@ -352,7 +356,7 @@ __global__ __launch_bounds__(NW* kWave) void k_seg_count(const SegDesc* __restri
             if (it[t].ntiles > 1) d[t] = desc[it[t].seg];
         __syncthreads();  // hist reuse
         if (!PACKED) {
-            for (int c = lane; c < MC; c += kWave)
+            for (int c = lane; c < HW; c += kWave)
 #pragma unroll
                 for (uint32_t t = 0; t < P; ++t) hist[t][wave][c] = 0;
             __builtin_amdgcn_wave_barrier();
@@ -381,7 +385,7 @@ __global__ __launch_bounds__(NW* kWave) void k_seg_count(const SegDesc* __restri
                 if (full || idx < d[t].len) {
                     const uint32_t c = dis(v[t][k]);
                     if (PACKED) lc.add(c);
-                    else atomicAdd(&hist[t][wave][c], 1u);
+                    else atomicAdd(&hist[t][wave][HR::at(c, lane)], 1u);
                 }
             }
             if (PACKED) {
@@ -395,7 +399,14 @@ __global__ __launch_bounds__(NW* kWave) void k_seg_count(const SegDesc* __restri
             for (uint32_t c = threadIdx.x; c < d[t].m; c += THREADS) {
                 uint32_t s = 0;
 #pragma unroll
-                for (int w = 0; w < NW; ++w) s += hist[t][w][c];
+                for (int w = 0; w < NW; ++w) {
+                    if (PACKED) {
+                        s += hist[t][w][c];
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < HR::kRep; ++k) s += hist[t][w][c * HR::kRep + k];
+                    }
+                }
                 counts[(size_t)c * ld + it[t].row] = s;
             }
         }

@@ -150,9 +150,10 @@ __global__ __launch_bounds__(NW* kWave) void k_src_count(
     constexpr int MC = 1 << NBITS;
     constexpr int TILE = NW * kWave * R;
     constexpr int THREADS = NW * kWave;
-    constexpr bool PACKED = LaneCounts<NBITS>::kPacked;
+    constexpr bool PACKED = count_packed<NBITS, Discr>();
     static_assert(R <= 255, "byte counters per lane");
-    __shared__ uint32_t hist[2][NW][MC];
+    using HR = HistRep<MC>;
+    __shared__ uint32_t hist[2][NW][PACKED ? MC : HR::kWords];
     const uint32_t lane = lane_id();
     const uint32_t wave = wave_id();
     const uint32_t chunk = xcd_tile(blockIdx.x, gridDim.x);
@@ -160,9 +161,9 @@ __global__ __launch_bounds__(NW* kWave) void k_src_count(
     const uint32_t t0 = chunk * CT;
     const uint32_t t1 = min(t0 + (uint32_t)CT, T);
     for (uint32_t t = t0; t < t1; ++t) {
-        uint32_t (*h)[MC] = hist[(t - t0) & 1u];
+        uint32_t (*h)[PACKED ? MC : HR::kWords] = hist[(t - t0) & 1u];
         if (!PACKED) {
-            for (int c = lane; c < MC; c += kWave) h[wave][c] = 0;
+            for (int c = lane; c < HR::kWords; c += kWave) h[wave][c] = 0;
             __builtin_amdgcn_wave_barrier();
         }
         LaneCounts<NBITS> lc;
@@ -176,7 +177,7 @@ __global__ __launch_bounds__(NW* kWave) void k_src_count(
             if (full || idx < n) {
                 const uint32_t c = dis(v[k]);
                 if (PACKED) lc.add(c);
-                else atomicAdd(&h[wave][c], 1u);
+                else atomicAdd(&h[wave][HR::at(c, lane)], 1u);
             }
         }
         if (PACKED) {
@@ -187,7 +188,14 @@ __global__ __launch_bounds__(NW* kWave) void k_src_count(
         for (uint32_t c = threadIdx.x; c < m; c += THREADS) {
             uint32_t s = 0;
 #pragma unroll
-            for (int w = 0; w < NW; ++w) s += h[w][c];
+            for (int w = 0; w < NW; ++w) {
+                if (PACKED) {
+                    s += h[w][c];
+                } else {
+#pragma unroll
+                    for (int k = 0; k < HR::kRep; ++k) s += h[w][c * HR::kRep + k];
+                }
+            }
             counts[(size_t)c * ld + t] = s;
         }
     }
