@@ -63,6 +63,10 @@ struct SegArgs {
     uint32_t grid_cap;  // no more workgroups than the queue can hold
     uint32_t item_cap;  // items the workspace's queue has room for
     uint32_t num_cu;
+    // False when no segment can span more than one tile (every length known
+    // on the host to be at most the tile): no count rows, so the count and
+    // scan launches are skipped.
+    bool rows = true;
 };
 
 // Workgroups of `kernel` resident on one CU (the occupancy API), at least 1.
@@ -104,10 +108,13 @@ inline void launch_cfg(const SegArgs& a, hipStream_t s) {
     static const int count_per_cu = resident_per_cu(k_seg_count<NW, R, NBITS, MASK>, NW * kWave);
     static const int part_per_cu =
         resident_per_cu(k_seg_partition<NW, R, NBITS, MASK, STAGED>, NW * kWave);
-    hipLaunchKernelGGL((k_seg_count<NW, R, NBITS, MASK>), dim3(persistent_grid(count_per_cu, a)),
-                       dim3(NW * kWave), 0, s, a.desc, a.items, a.plan, a.counts, a.ld);
-    launch_tile_scan(a.counts, a.prefix, a.max_rows + 1u, a.ld, a.mrows, nullptr, nullptr,
-                     &a.plan->n_rows, s);
+    if (a.rows) {
+        hipLaunchKernelGGL((k_seg_count<NW, R, NBITS, MASK>),
+                           dim3(persistent_grid(count_per_cu, a)), dim3(NW * kWave), 0, s,
+                           a.desc, a.items, a.plan, a.counts, a.ld);
+        launch_tile_scan(a.counts, a.prefix, a.max_rows + 1u, a.ld, a.mrows, nullptr, nullptr,
+                         &a.plan->n_rows, s);
+    }
     hipLaunchKernelGGL((k_seg_partition<NW, R, NBITS, MASK, STAGED>),
                        dim3(persistent_grid(part_per_cu, a)), dim3(NW * kWave), 0, s, a.desc,
                        a.items, a.plan, a.prefix, a.ld);

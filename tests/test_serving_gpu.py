@@ -119,6 +119,49 @@ def test_batch_power_of_two_mix_uses_mask_kernels(ctx):
         assert first_diff(from_dev(d_out[i]), ref) < 0
 
 
+# (waves per workgroup, elements per lane) of the 24 segmented / batch
+# candidates, in MDISCR_SEG_CONFIG order.
+SEG_CANDIDATES = [(4, 16), (4, 16), (8, 8), (8, 8), (4, 8), (8, 16), (16, 4), (4, 32),
+                  (4, 4), (8, 4), (8, 32), (16, 8), (16, 16), (16, 32), (4, 4), (4, 8),
+                  (4, 32), (8, 4), (8, 16), (8, 32), (16, 4), (16, 8), (16, 16), (16, 32)]
+
+
+@pytest.mark.parametrize("cfg", list(range(24)))
+def test_batch_of_one_tile_requests(ctx, cfg):
+    """Batches whose requests each fit one tile of the configuration launch no
+    count or scan: requests at 0, 1 and the tile and wave-share boundaries,
+    then the same batch with one request a key past the tile (count rows
+    again).  Outputs and class counts (pre-filled with -1) equal the oracle."""
+    import os
+    import torch
+    import mdiscriminator_amd as md
+    nw, r = SEG_CANDIDATES[cfg]
+    tile, wt = nw * 64 * r, 64 * r
+    for extra in ([], [tile + 1]):
+        sizes = [0, 1, 63, wt - 1, wt, wt + 1, tile - 1, tile, 5, tile] + extra
+        ms = [(2, 3, 4, 7, 8, 16, 64)[i % 7] for i in range(len(sizes))]
+        discrs = [md.Mask(m) if m & (m - 1) == 0 else md.Mod(m) for m in ms]
+        xs = [W.uniform_u31(n, 9000 + 97 * cfg + i) for i, n in enumerate(sizes)]
+        print(f"one-tile batch cfg={cfg} tile={tile} extra={extra}", flush=True)
+        d_in = [to_dev(x) if len(x) else torch.empty(0, dtype=torch.int32, device="cuda:0")
+                for x in xs]
+        d_out = [torch.full_like(t, -1) for t in d_in]
+        d_cnt = [torch.full((d.m,), -1, dtype=torch.int32, device="cuda:0") for d in discrs]
+        os.environ["MDISCR_SEG_CONFIG"] = str(cfg)
+        try:
+            ctx.partition_batch([(d_in[i], d_out[i], discrs[i], d_cnt[i])
+                                 for i in range(len(xs))])
+            torch.cuda.synchronize()
+        finally:
+            del os.environ["MDISCR_SEG_CONFIG"]
+        for i, x in enumerate(xs):
+            d = discrs[i]
+            ref, ref_cnt = oracle.seq_mdiscr(x, d.m, oracle.MASK if d.kind == md.DISCR_MASK
+                                             else oracle.MOD)
+            assert first_diff(from_dev(d_out[i]), ref) < 0, f"request {i} (n={len(x)})"
+            assert np.array_equal(from_dev(d_cnt[i]), ref_cnt), f"request {i} (n={len(x)})"
+
+
 def test_server_results_same_alone_and_in_mix():
     import torch
     import mdiscriminator_amd as md
