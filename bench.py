@@ -110,6 +110,33 @@ def cpu_baseline(quick=False):
     return head
 
 
+def read_ceiling(d_in, stream):
+    """The practical ceiling of the count pass: one streaming read of the same
+    1 GiB with the count's tile shape and 16-byte non-temporal loads
+    (libclkprobe.so), median of 20 event-timed launches on `stream` (a torch
+    stream)."""
+    import ctypes
+    import torch
+    probe = ctypes.CDLL(os.path.join(ROOT, "mdiscriminator_amd", "libclkprobe.so"))
+    probe.mdiscr_read_probe.restype = ctypes.c_int
+    probe.mdiscr_read_probe.argtypes = [ctypes.c_void_p, ctypes.c_ulonglong, ctypes.c_void_p,
+                                        ctypes.c_void_p]
+    n = d_in.numel()
+    sink = torch.zeros(n // 8192, dtype=torch.int32, device=d_in.device)
+    times = []
+    for i in range(23):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        if probe.mdiscr_read_probe(d_in.data_ptr(), n, sink.data_ptr(), stream.cuda_stream) != 0:
+            raise RuntimeError("read probe launch failed")
+        e1.record(stream)
+        torch.cuda.synchronize()
+        if i >= 3:
+            times.append(e0.elapsed_time(e1))
+    return sorted(times)[len(times) // 2]
+
+
 def first_steps(ctx, step, stream, steps):
     """The first `steps` steps, each preceded by a shader-clock probe."""
     import ctypes
@@ -305,6 +332,7 @@ def main():
         copy_ms.append(e0.elapsed_time(e1))
     copy_ms = sorted(copy_ms)[len(copy_ms) // 2]
     copy_gbs = N * BYTES_PER_ELEM / (copy_ms * 1e-3) / 1e9
+    read_ms = read_ceiling(d_in, stream)
     traffic, traffic_note = None, "no counter profile"
     tpath = os.path.join(ROOT, "profiles", "traffic_partition_2p28_m8.json")
     if os.path.exists(tpath):
@@ -498,6 +526,15 @@ def main():
                          "ms": round(copy_ms, 5), "achieved": round(copy_gbs, 1), "unit": "GB/s",
                          "frac_of_peak": round(copy_gbs / PEAK_HBM_GBS, 4),
                          "partition_vs_copy": round(achieved / copy_gbs, 4)},
+        # the count pass against a streaming read of the same 1 GiB, and the
+        # step against that read plus the copy (the two passes' floor)
+        "read_ceiling": {"kernel": "streaming read of the same 1 GiB, 16-byte loads in the "
+                                   "count's 8192-word tiles (libclkprobe.so)",
+                         "ms": round(read_ms, 5),
+                         "achieved": round(N * 4 / (read_ms * 1e-3) / 1e9, 1), "unit": "GB/s",
+                         "count_vs_read": (round(read_ms / kt["chunk_count"], 4)
+                                           if kt.get("chunk_count") else None),
+                         "step_vs_read_plus_copy": round((read_ms + copy_ms) / ms_step, 4)},
         "kernels_ms": {k: round(v, 5) for k, v in kt.items() if v},
         "zipf_variant": {"value": round(N * args.steps / dtz / 1e9, 3), "unit": "Gelem/s",
                          "kernels_ms": {k: round(v, 5) for k, v in ktz.items() if v}},

@@ -1,5 +1,6 @@
-// clock_probe.hip -- measurement support for bench.py --first-steps: the
-// shader clock seen by one lane, sampled between steps of a workload.
+// clock_probe.hip -- measurement support for bench.py: the shader clock seen
+// by one lane, sampled between steps of a workload (--first-steps), and the
+// device's streaming-read rate (the ceiling of a count pass).
 //
 // clk_probe: lane 0 of one workgroup counts shader-clock ticks (clock64)
 // over a fixed span of the constant-rate wall clock (wall_clock64) and stores
@@ -28,4 +29,34 @@ extern "C" int mdiscr_clk_probe(void* out, unsigned slot, unsigned span, void* s
 // Ticks per second of wall_clock64 on device 0, in kHz.
 extern "C" int mdiscr_wall_clock_khz(int* khz) {
     return (int)hipDeviceGetAttribute(khz, hipDeviceAttributeWallClockRate, 0);
+}
+
+// read_probe: each workgroup of 8 waves reads one tile of 8192 consecutive
+// words with 16-byte non-temporal loads (the count pass's tile shape and load
+// form) and folds them into one word, stored only when it equals a constant
+// no uniform input is likely to give, so the loads cannot be dropped.
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(512) void read_probe(const u32x4* __restrict__ in,
+                                                  unsigned* __restrict__ out) {
+    const u32x4* lp = in + (size_t)blockIdx.x * 2048 + threadIdx.x;
+    u32x4 acc = {0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc ^= __builtin_nontemporal_load(lp + k * 512);
+    const unsigned x = acc.x ^ acc.y ^ acc.z ^ acc.w;
+    if (x == 0x9E3779B9u) out[blockIdx.x] = x;
+}
+
+// Reads n_words (a multiple of 8192) words at `in` (16-byte aligned) once;
+// out must hold n_words / 8192 words.  -1 for a size or alignment it does
+// not take.
+extern "C" int mdiscr_read_probe(const void* in, unsigned long long n_words, void* out,
+                                 void* stream) {
+    if (n_words == 0 || n_words % 8192 || ((unsigned long long)in & 15) ||
+        n_words / 8192 > 0x7FFFFFFFull)
+        return -1;
+    hipLaunchKernelGGL(read_probe, dim3((unsigned)(n_words / 8192)), dim3(512), 0,
+                       (hipStream_t)stream, static_cast<const u32x4*>(in),
+                       static_cast<unsigned*>(out));
+    return (int)hipGetLastError();
 }

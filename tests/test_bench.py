@@ -29,6 +29,9 @@ def test_bench_line_is_complete():
     assert rf["bound"] == "hbm" and 0 < rf["frac"] < 1 and rf["achieved"] > 0
     cc = d["copy_ceiling"]   # the same buffers copied on the same box
     assert cc["achieved"] > 0 and 0 < cc["partition_vs_copy"] < 1.5
+    rc = d["read_ceiling"]   # a streaming read of the same input on the same box
+    assert rc["achieved"] > 0 and 0 < rc["count_vs_read"] < 1.5
+    assert 0 < rc["step_vs_read_plus_copy"] < 1.5
     for m in ("m16", "m32", "m64"):
         assert d["soa_records_2p26"][m]["partition_frac"] > 0
     for name, line in d["general"].items():
