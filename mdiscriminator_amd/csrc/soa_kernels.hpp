@@ -203,18 +203,20 @@ __global__ __launch_bounds__(NW* kWave) void k_src_count(
 
 // ---------------------------------------------------------------------------
 // Where the tile's slot s = k*THREADS + tid goes: dpos[class of s] + s.  Kept
-// in registers (one per slot the thread writes), or, with LDS, looked up again
-// for every column from the slot's class.  The lookup frees R registers
-// (107 -> 85 at 8 waves x 12 rounds) but costs two LDS reads per slot and
+// in registers (one per slot the thread writes), or, with LDS, in an LDS word
+// per slot that only the thread owning the slot writes and reads.  LDS frees
+// R registers (107 -> 85 at 8 waves x 12 rounds) for one LDS read per slot and
 // column: it pays only for the select source (2^26 (uint, uchar, bool)
 // records, m=8: partition 0.267 -> 0.254 ms; 4-byte key columns 2-5% slower).
+// Against looking the class and dpos up again for every column (two LDS
+// reads), the LDS word took the select's partition 0.234 -> 0.222 ms at
+// m=32 and 0.245 -> 0.233 at m=64.
 template <int R, bool LDS>
 struct SlotDest {
     uint32_t reg[LDS ? 1 : R];
-    const uint8_t* cls;
-    const uint32_t* dpos;
+    const uint32_t* dslot;
     __device__ __forceinline__ uint32_t at(int k, uint32_t s) const {
-        if constexpr (LDS) return dpos[cls[s]] + s;
+        if constexpr (LDS) return dslot[s];
         else return reg[k];
     }
 };
@@ -281,6 +283,8 @@ __global__ __launch_bounds__(NW* kWave) void k_partition_soa(
     constexpr int THREADS = NW * kWave;
     __shared__ ST stage[TILE];           // widest column element
     __shared__ uint8_t cls[TILE];        // class of each LDS slot
+    constexpr bool kDestLds = std::is_same_v<Src, SelectSrc>;
+    __shared__ uint32_t dslot[kDestLds ? TILE : 1];  // slot destinations (own thread's)
     __shared__ uint32_t cursor[NW][MC];
     __shared__ uint32_t dpos[MC];
 
@@ -348,16 +352,14 @@ __global__ __launch_bounds__(NW* kWave) void k_partition_soa(
             cls[slot[r]] = (uint8_t)c;
         }
         __syncthreads();
-        constexpr bool kDestLds = std::is_same_v<Src, SelectSrc>;
         SlotDest<R, kDestLds> dest;
-        dest.cls = cls;
-        dest.dpos = dpos;
-        if constexpr (!kDestLds) {
+        dest.dslot = dslot;
 #pragma unroll
-            for (int k = 0; k < R; ++k) {
-                const uint32_t s = k * THREADS + threadIdx.x;
-                dest.reg[k] = (full || s < valid) ? dpos[cls[s]] + s : 0u;
-            }
+        for (int k = 0; k < R; ++k) {
+            const uint32_t s = k * THREADS + threadIdx.x;
+            const uint32_t d = (full || s < valid) ? dpos[cls[s]] + s : 0u;
+            if constexpr (kDestLds) dslot[s] = d;
+            else dest.reg[k] = d;
         }
 
         // 4. Every column through the staging buffer, the key column first:
