@@ -28,6 +28,8 @@
 // bytes.
 #pragma once
 
+#include <type_traits>
+
 #include "device_common.hpp"
 
 // Cache-policy switches for the plain kernels (1 = non-temporal hint).
@@ -163,7 +165,8 @@ struct LaneCounts {
 
 template <int NBITS, class Discr>
 constexpr bool count_packed() {
-    return LaneCounts<NBITS>::kPacked && !(MDISCR_MOD_ATOMIC_COUNT && CostlyDiscr<Discr>::value);
+    return LaneCounts<NBITS>::kPacked &&
+           !(MDISCR_MOD_ATOMIC_COUNT && std::is_same_v<Discr, DiscrMod>);
 }
 
 // The atomic count's per-wave histogram: kRep copies of each class's word,
