@@ -480,6 +480,12 @@ def main():
             lambda: ctx.partition_soa_select([tx, ty, tz], touts, md.Mask(8), 2, 0, 1,
                                              stream=stream),
             nt, 2 * 6, ["soa_count", "soa_scan", "soa_partition"], "soa_partition")
+        # above 16 classes the select runs the ballot-ranked record kernel
+        for ms_ in (32, 64):
+            general[f"select_tuples_2p26_m{ms_}"] = shape_line(
+                lambda ms_=ms_: ctx.partition_soa_select([tx, ty, tz], touts, md.Mask(ms_), 2, 0,
+                                                         1, stream=stream),
+                nt, 2 * 6, ["soa_count", "soa_scan", "soa_partition"], "soa_partition")
         del tx, ty, tz, touts
 
     seg = serve = None
