@@ -11,7 +11,9 @@ into m = 8 classes (class = x & 7), inputs resident in HBM, through the C ABI
   cpu_baseline  the reference's own SeqMdiscr (compiled from source into
                 oracle/_ref) on the same 2^28 values, on this host, with the
                 other configurations' CPU baselines beside it
-plus the skewed (Zipf 1.5) variant, the per-kernel breakdown, and the C host
+plus the two passes' practical floors on the same box (copy_ceiling: an
+elementwise copy of the same buffers; read_ceiling: one streaming read of the
+input), the skewed (Zipf 1.5) variant, the per-kernel breakdown, and the C host
 drivers' results: the batched segmented workload (10,000 lognormal segments,
 one graph-replayed launch vs one plain launch per segment) and the serving
 workload (Poisson arrivals; continuous-batching server vs plain per-request
