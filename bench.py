@@ -351,7 +351,8 @@ def main():
     sweep = {}
     if not args.no_drivers:
         for label, dis in (("m2", md.Mask(2)), ("m4", md.Mask(4)), ("m16", md.Mask(16)),
-                           ("m32", md.Mask(32)), ("m64", md.Mask(64)), ("mod7", md.Mod(7))):
+                           ("m32", md.Mask(32)), ("m64", md.Mask(64)), ("mod7", md.Mod(7)),
+                           ("mod33", md.Mod(33))):
             def sweep_step():
                 ctx.partition_u32(d_in, d_out, dis, stream=stream)
             dt_m = time_steps(ctx, sweep_step, 10, 2)

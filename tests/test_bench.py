@@ -36,7 +36,7 @@ def test_bench_line_is_complete():
         assert d["soa_records_2p26"][m]["partition_frac"] > 0
     for name, line in d["general"].items():
         assert line["partition_frac"] and line["partition_frac"] > 0, name
-    for m in ("m2", "m4", "m16", "m32", "m64", "mod7"):
+    for m in ("m2", "m4", "m16", "m32", "m64", "mod7", "mod33"):
         assert 0 < d["plain_across_m"][m]["partition_frac"] < 1, m
     for k in ("n100", "n4096"):
         assert d["small_calls"][k]["us_per_call"] > 0
