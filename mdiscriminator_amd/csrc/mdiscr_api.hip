@@ -403,6 +403,12 @@ mdiscr_status launch_plain_quad(mdiscr_ctx* ctx, const uint32_t* in, uint32_t* o
 #ifndef MDISCR_LSD_CLS_R
 #define MDISCR_LSD_CLS_R 24
 #endif
+// ... and from 17 classes in tiles of 16 keys per lane (4096 keys): 2^28 keys,
+// partition x % 33 0.463 -> 0.432 ms, x % 48 0.471 -> 0.442 ms; at 9..16
+// classes the 6144-key count is faster (x % 13: call 0.601 against 0.605 ms).
+#ifndef MDISCR_LSD_CLS_R_WIDE
+#define MDISCR_LSD_CLS_R_WIDE 16
+#endif
 #ifndef MDISCR_LSD_R
 #define MDISCR_LSD_R 32
 #endif
@@ -426,11 +432,12 @@ constexpr bool use_lsd() {
                             (MDISCR_LSD_MOD16 && MDISCR_LSD_CLS && CostlyDiscr<Discr>::value))));
 }
 // The two-digit kernel's geometry for a discriminator.
-template <class Discr>
+template <class Discr, int NBITS>
 struct LsdGeom {
     static constexpr bool CLS = MDISCR_LSD_CLS && CostlyDiscr<Discr>::value;
     static constexpr int NW = MDISCR_LSD_NW;
-    static constexpr int R = CLS ? MDISCR_LSD_CLS_R : MDISCR_LSD_R;
+    static constexpr int R =
+        CLS ? (NBITS >= 5 ? MDISCR_LSD_CLS_R_WIDE : MDISCR_LSD_CLS_R) : MDISCR_LSD_R;
     static constexpr uint32_t TILE = (uint32_t)NW * kWave * R;
     static_assert(TILE >= kTile, "the workspace is sized by the smaller tile");
 };
@@ -439,7 +446,7 @@ template <int NBITS, class Discr>
 mdiscr_status launch_plain_lsd(mdiscr_ctx* ctx, const uint32_t* in, uint32_t* out, uint32_t n,
                                Discr dis, uint32_t m, uint32_t* user_counts, uint64_t* offsets,
                                hipStream_t s, uint32_t* ws) {
-    using G = LsdGeom<Discr>;
+    using G = LsdGeom<Discr, NBITS>;
     constexpr int NW = G::NW, R = G::R;
     constexpr uint32_t TILE = G::TILE;
     constexpr int CNW = 8;
@@ -530,7 +537,7 @@ mdiscr_status launch_plain(mdiscr_ctx* ctx, const uint32_t* in, uint32_t* out, u
                            Discr dis, uint32_t m, uint32_t* user_counts, uint64_t* offsets,
                            hipStream_t s, uint32_t* ws) {
     if constexpr (use_lsd<NBITS, Discr>()) {
-        if (((uintptr_t)in & 15u) == 0 && n > LsdGeom<Discr>::TILE)
+        if (((uintptr_t)in & 15u) == 0 && n > LsdGeom<Discr, NBITS>::TILE)
             return launch_plain_lsd<NBITS>(ctx, in, out, n, dis, m, user_counts, offsets, s, ws);
     }
     if constexpr (NBITS <= 4 && MDISCR_PLAIN_QUAD != 0) {
